@@ -1,0 +1,263 @@
+/*
+ * pasc_hip.h -- C ABI of the MI355X (gfx950) FEM-H1 K-means gamma/theta path.
+ *
+ * This is the drop-in boundary for the reference's PETSc vector path.  Each entry point names
+ * the reference interface it replaces (paths relative to the PASC_inference checkout).  All
+ * functions return a pasc_status (0 = PASC_OK); on failure pasc_last_error() returns a
+ * thread-local message.  No exceptions cross the ABI.  Host buffers belong to the caller;
+ * device buffers belong to the library behind opaque handles, except for the two kernel-level
+ * entry points (pasc_project_simplex, pasc_h1_matvec) which work on caller-owned device memory.
+ *
+ * Layouts: the reference keeps gamma t-major with K innermost (idx = t*R*K + r*K + k,
+ * decomposition.h:488-492).  Internally the library stores gamma k-major ([K][T*R], each row
+ * padded to a multiple of 64 elements) so that stencil rows are coalesced; the t-major layout is
+ * accepted and returned at this boundary (PASC_LAYOUT_TMAJOR) and transposed on the device.
+ */
+#ifndef PASC_HIP_H
+#define PASC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    PASC_OK = 0,
+    PASC_ERR_ARG = 1,      /* invalid argument (sizes, null pointers, unsupported K) */
+    PASC_ERR_HIP = 2,      /* a HIP runtime call failed */
+    PASC_ERR_ALLOC = 3,    /* device or host allocation failed */
+    PASC_ERR_STATE = 4,    /* call order violated (e.g. solve before theta was set) */
+    PASC_ERR_NUMERIC = 5   /* NaN/Inf met in the iteration */
+} pasc_status;
+
+typedef enum { PASC_LAYOUT_TMAJOR = 0, PASC_LAYOUT_KMAJOR = 1 } pasc_layout;
+
+typedef enum {
+    PASC_FP64 = 0,        /* storage, stencil, projection and reductions in FP64 */
+    PASC_FP32_BULK = 1    /* FP32 storage and bulk arithmetic, FP64 reductions and scalars,
+                             finished by FP64 polish iterations that certify the KKT target */
+} pasc_precision;
+
+typedef enum { PASC_DTYPE_F64 = 0, PASC_DTYPE_F32 = 1 } pasc_dtype;
+
+/* stopping criteria (spgqpsolver.cpp:260-274) plus the natural KKT residual */
+enum {
+    PASC_STOP_DIFFF = 1u,          /* |f - f_old| < eps                  (spgqpsolver_stop_difff) */
+    PASC_STOP_NORMGP = 2u,         /* dd < eps                           (spgqpsolver_stop_normgp) */
+    PASC_STOP_NORMGP_NORMB = 4u,   /* dd < eps*||b||                     (..._stop_normgp_normb) */
+    PASC_STOP_ANORMGP = 8u,        /* dAd < eps                          (..._stop_Anormgp) */
+    PASC_STOP_ANORMGP_NORMB = 16u, /* dAd < eps*||b||                    (..._stop_Anormgp_normb) */
+    PASC_STOP_KKT = 32u            /* max|gamma - P(gamma - s*g)| <= kkt_tol, s = T (scalef) */
+};
+
+/* stop_reason values in pasc_spg_result */
+enum {
+    PASC_REASON_MAXIT = 0, PASC_REASON_DIFFF = 1, PASC_REASON_NORMGP = 2,
+    PASC_REASON_NORMGP_NORMB = 3, PASC_REASON_ANORMGP = 4, PASC_REASON_ANORMGP_NORMB = 5,
+    PASC_REASON_KKT = 6, PASC_REASON_STATIONARY = 7, PASC_REASON_NUMERIC = 8
+};
+
+typedef struct pasc_ctx pasc_ctx;
+typedef struct pasc_problem pasc_problem;
+typedef struct pasc_batch pasc_batch;
+
+/* Problem description (replaces TSData + Decomposition + GraphH1FEMModel construction:
+ * tsdata.cpp:151-193, decomposition.cpp:6-67, graphh1fem.cpp:7-65). */
+typedef struct {
+    int64_t T;                 /* time steps */
+    int64_t R;                 /* graph nodes (1 for a 1-D signal) */
+    int32_t K;                 /* clusters, 1..PASC_MAX_K */
+    int32_t xdim;              /* data dimension (reference: 1; xdim > 1 is an extension) */
+    int32_t scalef;            /* graphh1femmodel_scalef (default 1): A = eps/T*A0, b /= T */
+    int32_t precision;         /* pasc_precision */
+    const double *x;           /* host data, [T][R][xdim] */
+    const int64_t *nbr_ptr;    /* optional CSR adjacency of the R nodes (R+1 entries) */
+    const int32_t *nbr_idx;    /* neighbour ids */
+    const uint8_t *heldout;    /* optional [T*R] mask: held-out nodes carry no data term */
+    int32_t grid_width;        /* > 0: the graph is the 4-neighbour grid of width grid_width
+                                  (R = width*height); nbr_ptr/nbr_idx may then be NULL */
+} pasc_problem_desc;
+
+#define PASC_MAX_K 8
+#define PASC_MAX_M 64
+
+/* SPG-QP settings (spgqpsolver.h:19-33, set_settings_from_console :161-202) */
+typedef struct {
+    int32_t maxit;        /* default 1000 */
+    double eps;           /* default 1e-9 */
+    int32_t m;            /* non-monotone window, default 20, at most PASC_MAX_M */
+    double gamma;         /* default 0.9 */
+    double sigma1;        /* default 0 */
+    double sigma2;        /* default 1 */
+    double alphainit;     /* default 2 */
+    uint32_t stop_mask;   /* default PASC_STOP_DIFFF */
+    double kkt_tol;       /* default 1e-6 */
+    int32_t check_every;  /* iterations between host convergence polls (default 1) */
+    int32_t use_graph;    /* capture check_every iterations in a hipGraph (default 0) */
+    double fp32_switch;   /* FP32-bulk: hand over to FP64 once the FP32 KKT <= this
+                             (default 0 = decide by stagnation only) */
+    int32_t stall_window; /* FP32-bulk: iterations without a 2x KKT improvement that end the
+                             FP32 phase (default 64) */
+    int32_t polish_maxit; /* FP32-bulk: maximum FP64 polish iterations (default maxit) */
+} pasc_spg_params;
+
+typedef struct {
+    int32_t it;           /* SPG iterations of this solve (QPSolver::it_last) */
+    int32_t hessmult;     /* Hessian products (1 + it, as spgqpsolver.cpp:125,167) */
+    int32_t status;       /* pasc_status of this solve */
+    int32_t stop_reason;  /* PASC_REASON_* */
+    int32_t it_fp32;      /* iterations spent in the FP32 phase (FP32-bulk) */
+    int32_t it_fp64;      /* FP64 iterations (all of them for PASC_FP64) */
+    double fx;            /* final objective 1/2 x'Ax - b'x (FP64 reduction) */
+    double dd;            /* gP = dot(d,d) of the last direction */
+    double dAd;
+    double kkt;           /* natural KKT residual of the returned gamma (FP64 arithmetic) */
+    double normb;         /* ||b||_2 */
+    double time_s;        /* wall time of the solve on the device (hipEvent) */
+} pasc_spg_result;
+
+/* Outer alternation settings (tssolver.h:13-15) */
+typedef struct {
+    int32_t maxit;             /* default 300 */
+    double eps;                /* default 1e-6, |L - L_old| < eps */
+    int32_t theta_fixed;       /* TSSolver::set_solution_theta: theta is never updated */
+    int32_t skip_first_theta;  /* start with a gamma step from the given theta */
+} pasc_ts_params;
+
+typedef struct {
+    int32_t it;            /* outer iterations performed */
+    int32_t spg_it_sum;    /* QPSolver::it_sum over the alternation */
+    double L;              /* final model value (GraphH1FEMModel::get_L = gamma solver fx) */
+    double deltaL;
+    int32_t increased;     /* times L increased ("ERROR: objective function increased") */
+    double aic;            /* GraphH1FEMModel::get_aic: 2 log L + K */
+} pasc_ts_result;
+
+/* ---- errors / version ---- */
+const char *pasc_last_error(void);
+int pasc_version(void);
+
+/* ---- context (replaces Initialize<PetscVector>, initialize.cpp:11-57) ---- */
+int pasc_ctx_create(int device, pasc_ctx **out);
+int pasc_ctx_destroy(pasc_ctx *ctx);
+/* Use an external HIP stream (hipStream_t cast to void*), or NULL for the context's own. */
+int pasc_ctx_set_stream(pasc_ctx *ctx, void *stream);
+int pasc_ctx_synchronize(pasc_ctx *ctx);
+
+/* ---- problem (TSData/QPData/GraphH1FEMModel) ---- */
+int pasc_problem_create(pasc_ctx *ctx, const pasc_problem_desc *desc, pasc_problem **out);
+int pasc_problem_destroy(pasc_problem *p);
+/* TSSolver::set_solution_theta (tssolver.h:777-790) writes theta; theta is [K][xdim] */
+int pasc_set_theta(pasc_problem *p, const double *theta);
+int pasc_get_theta(pasc_problem *p, double *theta);
+/* TSData::load_gammavector (tsdata.cpp:548-627) / the gamma vector read-out */
+int pasc_set_gamma(pasc_problem *p, const double *gamma, int layout);
+int pasc_set_gamma_uniform(pasc_problem *p);
+int pasc_get_gamma(pasc_problem *p, double *gamma, int layout);
+/* GraphH1FEMModel::set_epssqr (graphh1fem.h:286-306) */
+int pasc_set_epssqr(pasc_problem *p, double epssqr);
+
+/* ---- the gamma step ----
+ * updatebeforesolve_gammasolver (graphh1fem.cpp:236-289) + SPGQPSolver<PetscVector>::solve
+ * (spgqpsolver.cpp:55-312) + updateaftersolve_gammasolver (:291-301).  b is fused on the fly. */
+void pasc_spg_default_params(pasc_spg_params *p);
+int pasc_gamma_solve(pasc_problem *p, const pasc_spg_params *params, pasc_spg_result *res);
+
+/* ---- the theta step: updatebeforesolve_thetasolver (graphh1fem.cpp:305-396), FP64 sums ---- */
+int pasc_theta_update(pasc_problem *p);
+
+/* ---- outer alternation: TSSolver<PetscVector>::solve (tssolver.h:491-715) ---- */
+void pasc_ts_default_params(pasc_ts_params *p);
+int pasc_ts_solve(pasc_problem *p, const pasc_ts_params *tp, const pasc_spg_params *sp,
+                  pasc_ts_result *res);
+
+/* ---- model read-outs ----
+ * objective f = 1/2 g'Ag - b'g of the current gamma, explicit FP64 (get_fx, spgqpsolver.cpp:316)
+ * and its KKT residual */
+int pasc_objective(pasc_problem *p, double *fx, double *kkt);
+/* Signal1DData::compute_abserr_reconstructed (signal1Ddata.cpp:133-191): ||sum theta_k gamma_k
+ * - solution||_1; solution is a host [T*R] array (xdim = 1) */
+int pasc_abserr_reconstructed(pasc_problem *p, const double *solution, double *abserr);
+/* argmax labels [T*R] (int32); ties resolve to the lowest k (cf. tsdata.cpp:195-235) */
+int pasc_get_labels(pasc_problem *p, int32_t *labels);
+
+/* ---- kernel-level entry points on caller-owned device memory ----
+ * SimplexFeasibleSet_Local<PetscVector>::project (simplex_local.cpp:37-65, kernel_project
+ * simplex_local.cu:70-139): project every K-block (t-major: K consecutive values; k-major:
+ * stride ld between the K values) of x in place.  K <= 32 uses one thread per block; larger K
+ * uses one wavefront per block. */
+int pasc_project_simplex(pasc_ctx *ctx, void *x_dev, int64_t nblocks, int32_t K, int dtype,
+                         int layout, int64_t ld);
+/* BlockGraphSparseMatrix<PetscVector>::matmult (blockgraphsparse.cpp:151-194): y = alpha*A0*x
+ * for the 1-D time graph (R = 1) or the 4-neighbour grid graph (T = 1, R = width*height).
+ * x, y are k-major device arrays with row stride ld. */
+int pasc_h1_matvec(pasc_ctx *ctx, const void *x_dev, void *y_dev, int64_t T, int64_t R,
+                   int32_t K, int64_t ld, double alpha, int32_t grid_width, int dtype);
+
+/* ---- batched epsilon scan (new capability; cf. the sequential scan test_signal1D.cpp:268-353) --
+ * n_series series (each its own T, K, data and theta), each solved for every epsilon in eps[]:
+ * n_series*n_eps independent gamma QPs in one graph-captured launch sequence with per-instance
+ * convergence masks.  Series data are host arrays; theta is [K_i] per series (xdim = 1). */
+typedef struct {
+    int64_t T;
+    int32_t K;
+    const double *x;
+    const double *theta;
+} pasc_series_desc;
+
+typedef struct {
+    int32_t it;
+    int32_t it_fp32;
+    int32_t it_fp64;
+    int32_t stop_reason;
+    int32_t status;
+    double fx;
+    double kkt;
+} pasc_instance_result;
+
+int pasc_batch_create(pasc_ctx *ctx, int32_t n_series, const pasc_series_desc *series,
+                      int32_t n_eps, const double *eps, int32_t precision, pasc_batch **out);
+int pasc_batch_destroy(pasc_batch *b);
+/* solve all instances from gamma0 = 1/K; results are per instance, index = s*n_eps + e */
+int pasc_batch_solve(pasc_batch *b, const pasc_spg_params *params, pasc_instance_result *res,
+                     double *time_s);
+int pasc_batch_get_gamma(pasc_batch *b, int32_t series, int32_t eps_index, double *gamma,
+                         int layout);
+
+/* ---- adjoint (new capability) ----
+ * Gradient of the held-out reconstruction loss  J = sum_{t in M} (x_t - sum_k theta_k gamma_kt)^2
+ * with respect to log(epssqr) and theta, through the converged gamma of the last
+ * pasc_gamma_solve (implicit differentiation on the active face of the simplex, projected CG).
+ * mask is [T*R] (1 = held out).  dtheta has K entries (xdim = 1). */
+int pasc_adjoint(pasc_problem *p, const uint8_t *mask, double cg_tol, int32_t cg_maxit,
+                 double *loss, double *dlogeps, double *dtheta, int32_t *cg_it);
+
+/* ---- autotuner ----
+ * Times the fused SPG pass for the problem's (K, precision, T) over the candidate
+ * (threads per block, time steps per thread) pairs and keeps the fastest for later solves.
+ * The choice is cached per context and written to cache_path when it is non-NULL. */
+int pasc_autotune(pasc_problem *p, const char *cache_path, int32_t *block, int32_t *chunk,
+                  double *us_per_pass);
+
+/* ---- synthetic data (deterministic SplitMix64 + Box-Muller generator) ---- */
+int pasc_generate_markov(uint64_t seed, int64_t T, int32_t K, const double *levels,
+                         double p_switch, double sigma, double *x, int32_t *labels);
+int pasc_generate_mask(uint64_t seed, int64_t n, double fraction, uint8_t *mask);
+int pasc_generate_rect_image(uint64_t seed, int32_t width, int32_t height, int32_t K,
+                             int32_t n_rect, double sigma, double *x, double *colours,
+                             int32_t *labels);
+/* per-series parameters of the uneven batch: T_i = round(10^U[lg_tmin,lg_tmax]), K_i uniform
+ * in {kmin..kmax}, levels U[0,level_max], p U[pmin,pmax], sigma U[smin,smax], series seed */
+int pasc_generate_batch_params(uint64_t seed, int32_t n_series, double lg_tmin, double lg_tmax,
+                               int32_t kmin, int32_t kmax, double level_max, double pmin,
+                               double pmax, double smin, double smax, int64_t *T, int32_t *K,
+                               double *levels /* n_series*PASC_MAX_K */, double *p_switch,
+                               double *sigma, uint64_t *series_seed);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

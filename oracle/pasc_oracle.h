@@ -1,0 +1,140 @@
+/*
+ * pasc_oracle.h -- FP64 CPU restatement of PASC_inference's FEM-H1 K-means gamma/theta path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product path: only the
+ * test-suite, the smoke check and the CPU-baseline leg of the benchmark may load it.  Nothing in
+ * pasc_inference_amd/ links, imports or calls it.
+ *
+ * Every routine works in the reference's own layout: gamma (and every vector of the gamma QP)
+ * is t-major with K innermost, idx = t*R*K + r*K + k (decomposition.h:488-492); data is
+ * idx = (t*R + r)*xdim + c; theta is theta[k*xdim + c] (graphh1fem.cpp:94).
+ *
+ * Parity pins (tests/test_oracle_*.py):
+ *   project      closed forms, shift invariance, permutation equivariance, projection KKT,
+ *                brute-force support enumeration for K <= 6, the Matlab restatement's algorithm.
+ *   matvec       exact integer rows of blockgraphsparse.cpp:60-104 via A*e_j columns, A*1 = 0,
+ *                symmetry, x^T A x = alpha * sum of squared differences.
+ *   spg          KKT certificate with exact multipliers, independent dense QP solves (scipy) on
+ *                tiny problems, the f-identity, feasibility, the non-monotone bound.
+ *   b / theta    closed forms; eps -> 0 gives Lloyd k-means.
+ *   ts solver    monotone L on the reference data fixture samplesignal.bin.
+ * The iterate sequence of the PETSc build itself is unpinned (PETSc/MPI reduction order).
+ */
+#ifndef PASC_ORACLE_H
+#define PASC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- simplex projection: simplex_local.h:153-249 (sort_bubble + project_sub) ---- */
+void or_sort_bubble(double *x, int n);
+void or_project_simplex(double *x, int64_t nblocks, int K, int nthreads);
+
+/* ---- H1 graph-time matrix: blockgraphsparse.cpp:48-109 (assembly rules), :151-194 (matmult)
+ * y = alpha * A0 * x, matrix-free.  nbr_ptr/nbr_idx is the CSR adjacency of the R graph nodes
+ * (NULL for the edge-free graph used by 1-D signals, graphh1fem.cpp:35-46). */
+void or_h1_matvec(const double *x, double *y, int64_t T, int64_t R, int K, double alpha,
+                  const int64_t *nbr_ptr, const int32_t *nbr_idx, int nthreads);
+
+/* Diagonal entry Wsum of A0 for node degree deg at time t (blockgraphsparse.cpp:57-73). */
+int or_h1_wsum(int64_t t, int64_t T, int deg);
+
+/* ---- linear term: graphh1fem.cpp:236-289 (updatebeforesolve_gammasolver)
+ * b[t,r,k] = -sum_c (x[t,r,c] - theta[k,c])^2 / Tscale   (Tscale = T if scalef else 1)
+ * heldout (may be NULL): nodes with heldout[t*R+r] != 0 get b = 0 (no data term). */
+void or_assemble_b(const double *x, const double *theta, double *b, int64_t T, int64_t R, int K,
+                   int xdim, int scalef, const uint8_t *heldout, int nthreads);
+
+/* ---- theta update: graphh1fem.cpp:305-396 (k-means branch :374-379)
+ * theta[k,c] = sum_t x[t,c]*gamma[t,k] / sum_t gamma[t,k], 0 when the sum is 0. */
+void or_theta_update(const double *x, const double *gamma, double *theta, int64_t T, int64_t R,
+                     int K, int xdim, const uint8_t *heldout, int nthreads);
+
+/* ---- natural KKT residual of the gamma QP (no counterpart in the reference):
+ * max_{t,k} | gamma - P_simplex(gamma - s*g) |, s = T for the 1/T-scaled objective. */
+double or_kkt_residual(const double *gamma, const double *g, int64_t nblocks, int K, double s,
+                       int nthreads);
+
+/* ---- SPG-QP: spgqpsolver.cpp:55-367, spg_fs.cpp:7-63, defaults spgqpsolver.h:19-33 ---- */
+typedef struct {
+    int maxit;            /* spgqpsolver_maxit, default 1000 */
+    double eps;           /* spgqpsolver_eps, default 1e-9 */
+    int m;                /* spgqpsolver_m, default 20 */
+    double gamma;         /* spgqpsolver_gamma, default 0.9 */
+    double sigma1;        /* spgqpsolver_sigma1, default 0 */
+    double sigma2;        /* spgqpsolver_sigma2, default 1 */
+    double alphainit;     /* spgqpsolver_alphainit, default 2 */
+    int stop_difff;       /* default 1 */
+    int stop_normgp;      /* default 0 */
+    int stop_normgp_normb;
+    int stop_Anormgp;
+    int stop_Anormgp_normb;
+    int stop_kkt;         /* natural KKT residual <= kkt_tol (new criterion) */
+    double kkt_tol;
+    double kkt_scale;     /* s of or_kkt_residual */
+    int nthreads;
+} or_spg_params;
+
+typedef struct {
+    int it;
+    int hessmult;
+    double fx;
+    double dd;            /* gP: last dot(d,d) */
+    double dAd;
+    double kkt;           /* residual at the returned iterate (always computed) */
+    double normb;
+    int stop_reason;      /* 0 maxit, 1 difff, 2 normgp, 3 normgp_normb, 4 Anormgp,
+                             5 Anormgp_normb, 6 kkt */
+} or_spg_result;
+
+void or_spg_default_params(or_spg_params *p);
+
+/* Solve min 1/2 x'Ax - b'x over the product of simplices, A = alpha*A0 (graph as above).
+ * x holds x0 on entry and the solution on exit (the reference aliases x0 == x). */
+int or_spg_solve(double *x, const double *b, int64_t T, int64_t R, int K, double alpha,
+                 const int64_t *nbr_ptr, const int32_t *nbr_idx, const or_spg_params *p,
+                 or_spg_result *res);
+
+/* ---- outer alternation: tssolver.h:491-715, model hooks graphh1fem.cpp ---- */
+typedef struct {
+    int maxit;            /* tssolver_maxit, default 300 */
+    double eps;           /* tssolver_eps, default 1e-6 */
+    int theta_fixed;      /* set_solution_theta (tssolver.h:777-790): never update theta */
+    int skip_first_theta; /* start with a gamma step from the given theta */
+    int scalef;           /* graphh1femmodel_scalef, default 1 */
+} or_ts_params;
+
+typedef struct {
+    int it;               /* outer iterations performed (reference: it at break) */
+    int spg_it_sum;
+    double L;
+    double deltaL;
+    int increased;        /* number of times L increased (reference prints an ERROR) */
+} or_ts_result;
+
+int or_ts_solve(const double *x, double *gamma, double *theta, int64_t T, int64_t R, int K,
+                int xdim, double epssqr, const int64_t *nbr_ptr, const int32_t *nbr_idx,
+                const uint8_t *heldout, const or_ts_params *tp, const or_spg_params *sp,
+                or_ts_result *res);
+
+/* abserr = || sum_k theta_k gamma_k - solution ||_1  (signal1Ddata.cpp:133-191, xdim = 1) */
+double or_abserr_reconstructed(const double *gamma, const double *theta,
+                               const double *solution, int64_t T, int64_t R, int K);
+
+/* objective 1/2 x'Ax - b'x evaluated explicitly */
+double or_objective(const double *x, const double *b, int64_t T, int64_t R, int K, double alpha,
+                    const int64_t *nbr_ptr, const int32_t *nbr_idx, int nthreads);
+
+/* STREAM-like copy bandwidth of the host (bytes read + written per second) */
+double or_host_copy_bandwidth(int64_t nbytes, int reps, int nthreads);
+
+int or_max_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

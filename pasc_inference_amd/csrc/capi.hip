@@ -1,0 +1,587 @@
+// C ABI of libpasc_hip.so (declared in include/pasc_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "device_common.h"
+
+namespace pasc {
+
+static thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+void clear_error() { g_err.clear(); }
+
+}  // namespace pasc
+
+using namespace pasc;
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(PASC_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));       \
+    } while (0)
+
+struct pasc_ctx {
+    Ctx c;
+};
+
+struct pasc_problem {
+    pasc_ctx *ctx = nullptr;
+    int64_t T = 0, R = 0;
+    int K = 0, xdim = 1, scalef = 1, precision = PASC_FP64;
+    int32_t grid_width = 0;
+    double epssqr = 1.0;
+    bool theta_set = false;
+    bool gamma_set = false;
+    bool solved = false;
+    uint8_t *d_heldout = nullptr;
+    std::vector<uint8_t> h_heldout;
+    std::vector<double> h_x;  // host copy of the data, [T][R][xdim] (for abserr etc.)
+    Engine e;
+};
+
+struct pasc_batch {
+    pasc_ctx *ctx = nullptr;
+    int n_series = 0, n_eps = 0;
+    std::vector<int64_t> T;
+    std::vector<int32_t> K;
+    Engine e;
+};
+
+static int use_device(pasc_ctx *ctx) {
+    HIP_TRY(hipSetDevice(ctx->c.device));
+    return PASC_OK;
+}
+
+extern "C" {
+
+const char *pasc_last_error(void) { return g_err.c_str(); }
+
+int pasc_version(void) { return 100; }
+
+int pasc_ctx_create(int device, pasc_ctx **out) {
+    clear_error();
+    if (!out) return fail(PASC_ERR_ARG, "ctx_create: out is NULL");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(PASC_ERR_ARG, "ctx_create: device %d of %d", device, n);
+    HIP_TRY(hipSetDevice(device));
+    pasc_ctx *c = new (std::nothrow) pasc_ctx();
+    if (!c) return fail(PASC_ERR_ALLOC, "ctx_create: out of host memory");
+    c->c.device = device;
+    hipStream_t s;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(PASC_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    c->c.stream = s;
+    c->c.own_stream = true;
+    int ncu = 0;
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
+    c->c.n_cu = ncu;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->c.arch = prop.gcnArchName;
+    *out = c;
+    return PASC_OK;
+}
+
+int pasc_ctx_destroy(pasc_ctx *ctx) {
+    if (!ctx) return PASC_OK;
+    (void)hipSetDevice(ctx->c.device);
+    if (ctx->c.own_stream && ctx->c.stream) (void)hipStreamDestroy((hipStream_t)ctx->c.stream);
+    delete ctx;
+    return PASC_OK;
+}
+
+int pasc_ctx_set_stream(pasc_ctx *ctx, void *stream) {
+    if (!ctx) return fail(PASC_ERR_ARG, "ctx is NULL");
+    if (ctx->c.own_stream && ctx->c.stream) (void)hipStreamDestroy((hipStream_t)ctx->c.stream);
+    if (stream) {
+        ctx->c.stream = stream;
+        ctx->c.own_stream = false;
+    } else {
+        hipStream_t s;
+        HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        ctx->c.stream = s;
+        ctx->c.own_stream = true;
+    }
+    return PASC_OK;
+}
+
+int pasc_ctx_synchronize(pasc_ctx *ctx) {
+    if (!ctx) return fail(PASC_ERR_ARG, "ctx is NULL");
+    HIP_TRY(hipStreamSynchronize((hipStream_t)ctx->c.stream));
+    return PASC_OK;
+}
+
+void pasc_spg_default_params(pasc_spg_params *p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->maxit = 1000;
+    p->eps = 1e-9;
+    p->m = 20;
+    p->gamma = 0.9;
+    p->sigma1 = 0.0;
+    p->sigma2 = 1.0;
+    p->alphainit = 2.0;
+    p->stop_mask = PASC_STOP_DIFFF;
+    p->kkt_tol = 1e-6;
+    p->check_every = 1;
+    p->use_graph = 0;
+    p->fp32_switch = 0.0;
+    p->stall_window = 64;
+    p->polish_maxit = 0;
+}
+
+void pasc_ts_default_params(pasc_ts_params *p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->maxit = 300;
+    p->eps = 1e-6;
+}
+
+static int check_params(const pasc_spg_params *p) {
+    if (!p) return fail(PASC_ERR_ARG, "params is NULL");
+    if (p->maxit < 0) return fail(PASC_ERR_ARG, "maxit < 0");
+    if (p->m < 1 || p->m > PASC_MAX_M) return fail(PASC_ERR_ARG, "m must be in [1, %d]", PASC_MAX_M);
+    if (!(p->sigma2 >= p->sigma1)) return fail(PASC_ERR_ARG, "sigma2 < sigma1");
+    if (!(p->alphainit > 0)) return fail(PASC_ERR_ARG, "alphainit must be > 0");
+    return PASC_OK;
+}
+
+static void fill_desc(pasc_problem *p, InstDesc &d) {
+    const int64_t n = p->grid_width > 0 ? p->R : p->T;
+    d.T = n;
+    d.ld = (n + 63) / 64 * 64;
+    d.goff = 0;
+    d.xoff = 0;
+    d.K = p->K;
+    d.xdim = p->xdim;
+    const double Tn = (double)p->T;  // the scalef denominator is the number of time steps
+    d.alpha = p->scalef ? p->epssqr / Tn : p->epssqr;
+    d.bscale = p->scalef ? 1.0 / Tn : 1.0;
+    d.s_kkt = p->scalef ? Tn : 1.0;
+    d.heldout = p->d_heldout;
+}
+
+int pasc_problem_create(pasc_ctx *ctx, const pasc_problem_desc *desc, pasc_problem **out) {
+    clear_error();
+    if (!ctx || !desc || !out) return fail(PASC_ERR_ARG, "problem_create: NULL argument");
+    if (desc->T < 1 || desc->R < 1) return fail(PASC_ERR_ARG, "T and R must be >= 1");
+    if (desc->K < 1 || desc->K > PASC_MAX_K) return fail(PASC_ERR_ARG, "K must be in [1, %d]", PASC_MAX_K);
+    if (desc->xdim < 1 || desc->xdim > 3) return fail(PASC_ERR_ARG, "xdim must be in [1, 3]");
+    if (!desc->x) return fail(PASC_ERR_ARG, "x is NULL");
+    if (desc->precision != PASC_FP64 && desc->precision != PASC_FP32_BULK)
+        return fail(PASC_ERR_ARG, "unknown precision %d", desc->precision);
+    const bool grid = desc->grid_width > 0;
+    if (grid) {
+        if (desc->T != 1 || desc->R % desc->grid_width != 0)
+            return fail(PASC_ERR_ARG, "grid graph needs T == 1 and R a multiple of grid_width");
+    } else {
+        if (desc->R != 1 || desc->nbr_ptr || desc->nbr_idx)
+            return fail(PASC_ERR_ARG, "only the 1-D time graph (R == 1) and the 4-neighbour grid "
+                                      "(grid_width > 0, T == 1) are supported on the device");
+    }
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    pasc_problem *p = new (std::nothrow) pasc_problem();
+    if (!p) return fail(PASC_ERR_ALLOC, "problem_create: out of host memory");
+    p->ctx = ctx;
+    p->T = desc->T;
+    p->R = desc->R;
+    p->K = desc->K;
+    p->xdim = desc->xdim;
+    p->scalef = desc->scalef;
+    p->precision = desc->precision;
+    p->grid_width = desc->grid_width;
+    const int64_t n = grid ? p->R : p->T;
+    p->h_x.assign(desc->x, desc->x + n * p->xdim);
+    if (desc->heldout) {
+        p->h_heldout.assign(desc->heldout, desc->heldout + n);
+        hipError_t e = hipMalloc(&p->d_heldout, (size_t)n);
+        if (e == hipSuccess) e = hipMemcpy(p->d_heldout, desc->heldout, (size_t)n, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            if (p->d_heldout) (void)hipFree(p->d_heldout);
+            delete p;
+            return fail(PASC_ERR_HIP, "heldout upload: %s", hipGetErrorString(e));
+        }
+    }
+    InstDesc d;
+    std::memset(&d, 0, sizeof(d));
+    fill_desc(p, d);
+    std::vector<InstDesc> descs{d};
+    rc = engine_alloc(p->e, &ctx->c, p->precision, grid ? STENCIL_GRID : STENCIL_TIME,
+                      p->grid_width, descs, d.ld * p->xdim);
+    if (rc == PASC_OK) {
+        // data rows: SoA, row c holds channel c
+        std::vector<double> rows((size_t)d.ld * p->xdim, 0.0);
+        for (int c = 0; c < p->xdim; c++)
+            for (int64_t i = 0; i < n; i++) rows[(size_t)c * d.ld + i] = desc->x[i * p->xdim + c];
+        rc = engine_upload_data(p->e, rows.data(), 0, (int64_t)rows.size());
+    }
+    if (rc) {
+        engine_free(p->e);
+        if (p->d_heldout) (void)hipFree(p->d_heldout);
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return PASC_OK;
+}
+
+int pasc_problem_destroy(pasc_problem *p) {
+    if (!p) return PASC_OK;
+    (void)hipSetDevice(p->ctx->c.device);
+    engine_free(p->e);
+    if (p->d_heldout) (void)hipFree(p->d_heldout);
+    delete p;
+    return PASC_OK;
+}
+
+int pasc_set_theta(pasc_problem *p, const double *theta) {
+    clear_error();
+    if (!p || !theta) return fail(PASC_ERR_ARG, "set_theta: NULL argument");
+    for (int i = 0; i < p->K * p->xdim; i++) {
+        if (!std::isfinite(theta[i])) return fail(PASC_ERR_ARG, "set_theta: theta[%d] is not finite", i);
+        p->e.h_desc[0].theta[i] = theta[i];
+    }
+    p->theta_set = true;
+    int rc = use_device(p->ctx);
+    if (rc) return rc;
+    return engine_update_desc(p->e);
+}
+
+int pasc_get_theta(pasc_problem *p, double *theta) {
+    if (!p || !theta) return fail(PASC_ERR_ARG, "get_theta: NULL argument");
+    for (int i = 0; i < p->K * p->xdim; i++) theta[i] = p->e.h_desc[0].theta[i];
+    return PASC_OK;
+}
+
+int pasc_set_gamma(pasc_problem *p, const double *gamma, int layout) {
+    clear_error();
+    if (!p || !gamma) return fail(PASC_ERR_ARG, "set_gamma: NULL argument");
+    if (layout != PASC_LAYOUT_TMAJOR && layout != PASC_LAYOUT_KMAJOR) return fail(PASC_ERR_ARG, "bad layout");
+    int rc = use_device(p->ctx);
+    if (rc) return rc;
+    rc = engine_set_gamma(p->e, 0, gamma, layout);
+    if (rc == PASC_OK) p->gamma_set = true;
+    return rc;
+}
+
+int pasc_set_gamma_uniform(pasc_problem *p) {
+    clear_error();
+    if (!p) return fail(PASC_ERR_ARG, "set_gamma_uniform: NULL argument");
+    int rc = use_device(p->ctx);
+    if (rc) return rc;
+    rc = engine_set_uniform(p->e, 0);
+    if (rc == PASC_OK) p->gamma_set = true;
+    return rc;
+}
+
+int pasc_get_gamma(pasc_problem *p, double *gamma, int layout) {
+    clear_error();
+    if (!p || !gamma) return fail(PASC_ERR_ARG, "get_gamma: NULL argument");
+    if (layout != PASC_LAYOUT_TMAJOR && layout != PASC_LAYOUT_KMAJOR) return fail(PASC_ERR_ARG, "bad layout");
+    if (!p->gamma_set) return fail(PASC_ERR_STATE, "get_gamma: gamma was never set");
+    int rc = use_device(p->ctx);
+    if (rc) return rc;
+    return engine_get_gamma(p->e, 0, gamma, layout);
+}
+
+int pasc_set_epssqr(pasc_problem *p, double epssqr) {
+    clear_error();
+    if (!p) return fail(PASC_ERR_ARG, "set_epssqr: NULL problem");
+    if (!(epssqr >= 0) || !std::isfinite(epssqr)) return fail(PASC_ERR_ARG, "epssqr must be finite and >= 0");
+    p->epssqr = epssqr;
+    InstDesc &d = p->e.h_desc[0];
+    d.alpha = p->scalef ? epssqr / (double)p->T : epssqr;
+    int rc = use_device(p->ctx);
+    if (rc) return rc;
+    return engine_update_desc(p->e);
+}
+
+int pasc_gamma_solve(pasc_problem *p, const pasc_spg_params *params, pasc_spg_result *res) {
+    clear_error();
+    if (!p) return fail(PASC_ERR_ARG, "gamma_solve: NULL problem");
+    int rc = check_params(params);
+    if (rc) return rc;
+    if (!p->theta_set) return fail(PASC_ERR_STATE, "gamma_solve: theta was never set");
+    rc = use_device(p->ctx);
+    if (rc) return rc;
+    if (!p->gamma_set) {
+        rc = engine_set_uniform(p->e, 0);
+        if (rc) return rc;
+        p->gamma_set = true;
+    }
+    pasc_spg_result r;
+    std::memset(&r, 0, sizeof(r));
+    rc = engine_solve(p->e, *params, &r, nullptr, nullptr);
+    if (res) *res = r;
+    if (rc == PASC_OK) p->solved = true;
+    return rc;
+}
+
+int pasc_theta_update(pasc_problem *p) {
+    clear_error();
+    if (!p) return fail(PASC_ERR_ARG, "theta_update: NULL problem");
+    if (!p->gamma_set) return fail(PASC_ERR_STATE, "theta_update: gamma was never set");
+    int rc = use_device(p->ctx);
+    if (rc) return rc;
+    rc = engine_theta_update(p->e, 0, nullptr);
+    if (rc == PASC_OK) p->theta_set = true;
+    return rc;
+}
+
+int pasc_ts_solve(pasc_problem *p, const pasc_ts_params *tp, const pasc_spg_params *sp,
+                  pasc_ts_result *res) {
+    clear_error();
+    if (!p || !tp) return fail(PASC_ERR_ARG, "ts_solve: NULL argument");
+    int rc = check_params(sp);
+    if (rc) return rc;
+    rc = use_device(p->ctx);
+    if (rc) return rc;
+    if (!p->gamma_set) {
+        rc = engine_set_uniform(p->e, 0);
+        if (rc) return rc;
+        p->gamma_set = true;
+    }
+    // tssolver.h:542-650
+    double L = std::numeric_limits<double>::max(), L_old, deltaL = L;
+    int it, spg_sum = 0, increased = 0;
+    for (it = 0; it < tp->maxit; it++) {
+        if (!tp->theta_fixed && !(it == 0 && tp->skip_first_theta)) {
+            rc = engine_theta_update(p->e, 0, nullptr);
+            if (rc) return rc;
+            p->theta_set = true;
+        }
+        if (!p->theta_set) return fail(PASC_ERR_STATE, "ts_solve: theta was never set");
+        pasc_spg_result r;
+        rc = engine_solve(p->e, *sp, &r, nullptr, nullptr);
+        if (rc) return rc;
+        spg_sum += r.it;
+        L_old = L;
+        L = r.fx;
+        deltaL = std::fabs(L - L_old);
+        if (L - L_old > 0) increased++;
+        if (deltaL < tp->eps) { it++; break; }
+    }
+    p->solved = true;
+    if (res) {
+        res->it = it;
+        res->spg_it_sum = spg_sum;
+        res->L = L;
+        res->deltaL = deltaL;
+        res->increased = increased;
+        res->aic = 2.0 * std::log(L) + p->K;  // graphh1fem.h:443-447
+    }
+    return PASC_OK;
+}
+
+int pasc_objective(pasc_problem *p, double *fx, double *kkt) {
+    clear_error();
+    if (!p) return fail(PASC_ERR_ARG, "objective: NULL problem");
+    if (!p->gamma_set) return fail(PASC_ERR_STATE, "objective: gamma was never set");
+    int rc = use_device(p->ctx);
+    if (rc) return rc;
+    return engine_eval(p->e, 0, fx, kkt);
+}
+
+int pasc_abserr_reconstructed(pasc_problem *p, const double *solution, double *abserr) {
+    clear_error();
+    if (!p || !solution || !abserr) return fail(PASC_ERR_ARG, "abserr: NULL argument");
+    if (p->xdim != 1) return fail(PASC_ERR_ARG, "abserr needs xdim == 1");
+    if (!p->gamma_set) return fail(PASC_ERR_STATE, "abserr: gamma was never set");
+    int rc = use_device(p->ctx);
+    if (rc) return rc;
+    return engine_abserr(p->e, 0, solution, abserr);
+}
+
+int pasc_get_labels(pasc_problem *p, int32_t *labels) {
+    clear_error();
+    if (!p || !labels) return fail(PASC_ERR_ARG, "labels: NULL argument");
+    if (!p->gamma_set) return fail(PASC_ERR_STATE, "labels: gamma was never set");
+    int rc = use_device(p->ctx);
+    if (rc) return rc;
+    return engine_labels(p->e, 0, labels);
+}
+
+int pasc_project_simplex(pasc_ctx *ctx, void *x_dev, int64_t nblocks, int32_t K, int dtype,
+                         int layout, int64_t ld) {
+    clear_error();
+    if (!ctx || (!x_dev && nblocks > 0)) return fail(PASC_ERR_ARG, "project: NULL argument");
+    if (nblocks < 0 || K < 1 || K > 65536) return fail(PASC_ERR_ARG, "project: bad sizes");
+    if (dtype != PASC_DTYPE_F32 && dtype != PASC_DTYPE_F64) return fail(PASC_ERR_ARG, "bad dtype");
+    if (layout != PASC_LAYOUT_TMAJOR && layout != PASC_LAYOUT_KMAJOR) return fail(PASC_ERR_ARG, "bad layout");
+    if (layout == PASC_LAYOUT_KMAJOR && ld < nblocks) return fail(PASC_ERR_ARG, "project: ld < nblocks");
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    return kernel_project(&ctx->c, x_dev, nblocks, K, dtype, layout, ld);
+}
+
+int pasc_h1_matvec(pasc_ctx *ctx, const void *x_dev, void *y_dev, int64_t T, int64_t R,
+                   int32_t K, int64_t ld, double alpha, int32_t grid_width, int dtype) {
+    clear_error();
+    if (!ctx || !x_dev || !y_dev) return fail(PASC_ERR_ARG, "matvec: NULL argument");
+    if (K < 1 || T < 1 || R < 1) return fail(PASC_ERR_ARG, "matvec: bad sizes");
+    if (grid_width > 0) {
+        if (T != 1 || R % grid_width) return fail(PASC_ERR_ARG, "matvec: grid needs T == 1, R % width == 0");
+        if (ld < R) return fail(PASC_ERR_ARG, "matvec: ld < R");
+    } else {
+        if (R != 1) return fail(PASC_ERR_ARG, "matvec: the time graph needs R == 1");
+        if (ld < T) return fail(PASC_ERR_ARG, "matvec: ld < T");
+    }
+    if (dtype != PASC_DTYPE_F32 && dtype != PASC_DTYPE_F64) return fail(PASC_ERR_ARG, "bad dtype");
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    return kernel_matvec(&ctx->c, x_dev, y_dev, T, R, K, ld, alpha, grid_width, dtype);
+}
+
+// ---- batch ----
+int pasc_batch_create(pasc_ctx *ctx, int32_t n_series, const pasc_series_desc *series,
+                      int32_t n_eps, const double *eps, int32_t precision, pasc_batch **out) {
+    clear_error();
+    if (!ctx || !series || !eps || !out || n_series < 1 || n_eps < 1)
+        return fail(PASC_ERR_ARG, "batch_create: bad arguments");
+    if (precision != PASC_FP64 && precision != PASC_FP32_BULK) return fail(PASC_ERR_ARG, "bad precision");
+    std::vector<InstDesc> descs;
+    std::vector<int64_t> xoff(n_series);
+    int64_t xo = 0, go = 0;
+    for (int s = 0; s < n_series; s++) {
+        const pasc_series_desc &sd = series[s];
+        if (sd.T < 1 || sd.K < 1 || sd.K > PASC_MAX_K || !sd.x || !sd.theta)
+            return fail(PASC_ERR_ARG, "batch_create: bad series %d", s);
+        xoff[s] = xo;
+        xo += (sd.T + 63) / 64 * 64;
+    }
+    for (int s = 0; s < n_series; s++) {
+        const pasc_series_desc &sd = series[s];
+        for (int j = 0; j < n_eps; j++) {
+            if (!(eps[j] >= 0) || !std::isfinite(eps[j])) return fail(PASC_ERR_ARG, "bad eps[%d]", j);
+            InstDesc d;
+            std::memset(&d, 0, sizeof(d));
+            d.T = sd.T;
+            d.ld = (sd.T + 63) / 64 * 64;
+            d.goff = go;
+            d.xoff = xoff[s];
+            d.K = sd.K;
+            d.xdim = 1;
+            d.alpha = eps[j] / (double)sd.T;
+            d.bscale = 1.0 / (double)sd.T;
+            d.s_kkt = (double)sd.T;
+            d.heldout = nullptr;
+            for (int k = 0; k < sd.K; k++) d.theta[k] = sd.theta[k];
+            go += (int64_t)d.K * d.ld;
+            descs.push_back(d);
+        }
+    }
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    pasc_batch *b = new (std::nothrow) pasc_batch();
+    if (!b) return fail(PASC_ERR_ALLOC, "batch_create: out of host memory");
+    b->ctx = ctx;
+    b->n_series = n_series;
+    b->n_eps = n_eps;
+    rc = engine_alloc(b->e, &ctx->c, precision, STENCIL_TIME, 0, descs, xo);
+    for (int s = 0; s < n_series && rc == PASC_OK; s++)
+        rc = engine_upload_data(b->e, series[s].x, xoff[s], series[s].T);
+    if (rc) {
+        engine_free(b->e);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return PASC_OK;
+}
+
+int pasc_batch_destroy(pasc_batch *b) {
+    if (!b) return PASC_OK;
+    (void)hipSetDevice(b->ctx->c.device);
+    engine_free(b->e);
+    delete b;
+    return PASC_OK;
+}
+
+int pasc_batch_solve(pasc_batch *b, const pasc_spg_params *params, pasc_instance_result *res,
+                     double *time_s) {
+    clear_error();
+    if (!b) return fail(PASC_ERR_ARG, "batch_solve: NULL batch");
+    int rc = check_params(params);
+    if (rc) return rc;
+    rc = use_device(b->ctx);
+    if (rc) return rc;
+    rc = engine_fill_uniform(b->e);
+    if (rc) return rc;
+    return engine_solve(b->e, *params, nullptr, res, time_s);
+}
+
+int pasc_batch_get_gamma(pasc_batch *b, int32_t series, int32_t eps_index, double *gamma,
+                         int layout) {
+    clear_error();
+    if (!b || !gamma) return fail(PASC_ERR_ARG, "batch_get_gamma: NULL argument");
+    if (series < 0 || series >= b->n_series || eps_index < 0 || eps_index >= b->n_eps)
+        return fail(PASC_ERR_ARG, "batch_get_gamma: index out of range");
+    int rc = use_device(b->ctx);
+    if (rc) return rc;
+    return engine_get_gamma(b->e, series * b->n_eps + eps_index, gamma, layout);
+}
+
+int pasc_adjoint(pasc_problem *p, const uint8_t *mask, double cg_tol, int32_t cg_maxit,
+                 double *loss, double *dlogeps, double *dtheta, int32_t *cg_it) {
+    clear_error();
+    if (!p || !mask || !loss || !dlogeps || !dtheta) return fail(PASC_ERR_ARG, "adjoint: NULL argument");
+    if (p->xdim != 1 || p->grid_width > 0) return fail(PASC_ERR_ARG, "adjoint: 1-D signals (xdim == 1) only");
+    if (!p->solved) return fail(PASC_ERR_STATE, "adjoint: solve the gamma problem first");
+    int rc = use_device(p->ctx);
+    if (rc) return rc;
+    int it = 0;
+    rc = engine_adjoint(p->e, 0, mask, cg_tol, cg_maxit, loss, dlogeps, dtheta, &it);
+    if (cg_it) *cg_it = it;
+    return rc;
+}
+
+int pasc_autotune(pasc_problem *p, const char *cache_path, int32_t *block, int32_t *chunk,
+                  double *us_per_pass) {
+    clear_error();
+    if (!p) return fail(PASC_ERR_ARG, "autotune: NULL problem");
+    if (p->grid_width > 0) return fail(PASC_ERR_ARG, "autotune: the grid path has a fixed shape");
+    if (!p->theta_set) return fail(PASC_ERR_STATE, "autotune: theta was never set");
+    int rc = use_device(p->ctx);
+    if (rc) return rc;
+    int bl = 0, ch = 0;
+    double us = 0;
+    rc = engine_autotune(p->e, &bl, &ch, &us);
+    if (rc) return rc;
+    if (block) *block = bl;
+    if (chunk) *chunk = ch;
+    if (us_per_pass) *us_per_pass = us;
+    if (cache_path) {
+        FILE *f = fopen(cache_path, "a");
+        if (f) {
+            fprintf(f, "{\"arch\": \"%s\", \"K\": %d, \"precision\": %d, \"T\": %lld, \"block\": %d, \"chunk\": %d, \"us_per_pass\": %.3f}\n",
+                    p->ctx->c.arch.c_str(), p->K, p->precision, (long long)p->T, bl, ch, us);
+            fclose(f);
+        }
+    }
+    return PASC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,150 @@
+// Device-side building blocks shared by the HIP translation units of libpasc_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pasc_internal.h"
+
+namespace pasc {
+
+constexpr int WAVE = 64;
+
+template <typename S> struct V16;
+template <> struct V16<float> {
+    using type = float4;
+    static constexpr int n = 4;
+    __device__ static inline float get(const type &v, int i) {
+        return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
+    }
+    __device__ static inline void set(type &v, int i, float a) {
+        if (i == 0) v.x = a; else if (i == 1) v.y = a; else if (i == 2) v.z = a; else v.w = a;
+    }
+};
+template <> struct V16<double> {
+    using type = double2;
+    static constexpr int n = 2;
+    __device__ static inline double get(const type &v, int i) { return i == 0 ? v.x : v.y; }
+    __device__ static inline void set(type &v, int i, double a) {
+        if (i == 0) v.x = a; else v.y = a;
+    }
+};
+
+template <typename C> __device__ inline C cmax(C a, C b) { return a > b ? a : b; }
+
+// Euclidean projection of one K-vector (in registers) onto the unit simplex
+// {y : sum y = 1, y >= 0}; replaces SimplexFeasibleSet_Local::project_sub
+// (simplex_local.h:176-249) with a sort-free threshold search:
+//   * exactly feasible input is returned unchanged (the reference's early exit);
+//   * tau is bracketed in [max(y) - 1, max(y)) (phi(tau) = sum max(y - tau, 0) - 1 changes sign
+//     there); NB bisection steps shrink the bracket, which matters for larger K;
+//   * the exact threshold is then the fixed point tau = (sum_{y > tau} y - 1) / |{y > tau}|
+//     iterated from the lower bracket end: the support only shrinks and at most K sweeps are
+//     needed, so the result is the same threshold as Chen & Ye's sorted search.
+template <int K, typename C>
+__device__ __forceinline__ void project_simplex(C (&y)[K]) {
+    C s = C(0);
+    bool neg = false;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        s += y[k];
+        neg |= (y[k] < C(0));
+    }
+    if (!neg && s == C(1)) return;
+    C mx = y[0];
+#pragma unroll
+    for (int k = 1; k < K; k++) mx = cmax(mx, y[k]);
+    C tau = mx - C(1);
+    constexpr int NB = K > 4 ? 4 : 0;
+    if (NB > 0) {
+        C hi = mx;
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            const C mid = C(0.5) * (tau + hi);
+            C phi = C(-1);
+#pragma unroll
+            for (int k = 0; k < K; k++) phi += cmax(y[k] - mid, C(0));
+            if (phi >= C(0)) tau = mid; else hi = mid;
+        }
+    }
+    int cnt_prev = -1;
+#pragma unroll
+    for (int it = 0; it < K; it++) {
+        C sum = C(0);
+        int cnt = 0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const bool in = y[k] > tau;
+            sum += in ? y[k] : C(0);
+            cnt += in ? 1 : 0;
+        }
+        if (cnt == cnt_prev || cnt == 0) break;
+        cnt_prev = cnt;
+        tau = (sum - C(1)) / C(cnt);
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) y[k] = cmax(y[k] - tau, C(0));
+}
+
+// Wavefront reductions (64 lanes).
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o, WAVE);
+        v = (w > v || w != w) ? w : v;
+    }
+    return v;
+}
+__device__ inline int wave_sum_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+// Block reduction of NPART partials (sum for the first 4, max for the last).  Thread 0 returns
+// the result in v.  lds must hold NPART * (NT/64) doubles.
+template <int NT>
+__device__ inline void block_reduce_part(double (&v)[NPART], double *lds) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int w = threadIdx.x / WAVE;
+#pragma unroll
+    for (int i = 0; i < NPART - 1; i++) v[i] = wave_sum(v[i]);
+    v[NPART - 1] = wave_max(v[NPART - 1]);
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < NPART; i++) lds[w * NPART + i] = v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < NT / WAVE; q++) {
+#pragma unroll
+            for (int i = 0; i < NPART - 1; i++) v[i] += lds[q * NPART + i];
+            const double m = lds[q * NPART + NPART - 1];
+            if (m > v[NPART - 1] || m != m) v[NPART - 1] = m;
+        }
+    }
+    __syncthreads();
+}
+
+// Tile enumeration over the active instances of a K group (see engine scan kernel).
+struct GroupView {
+    const int32_t *ids;
+    const int32_t *off;
+    const int32_t *hint;
+    int32_t n;
+};
+
+__device__ inline void find_tile(const GroupView &g, int j, int &inst, int &chunk) {
+    int a = g.hint[j >> 6];
+    while (g.off[a + 1] <= j) a++;
+    inst = g.ids[a];
+    chunk = j - g.off[a];
+}
+
+}  // namespace pasc

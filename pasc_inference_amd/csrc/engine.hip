@@ -1,0 +1,800 @@
+// Host side of the SPG engine: buffers, kernel dispatch, the convergence loop (optionally
+// hipGraph-captured), FP32 -> FP64 polish, and the auxiliary device computations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "spg_kernels.h"
+#include "step_kernels.h"
+#include "util_kernels.h"
+
+namespace pasc {
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(PASC_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                              \
+    } while (0)
+
+static hipStream_t S_(Engine &e) { return (hipStream_t)e.ctx->stream; }
+
+static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+// ------------------------------------------------------------------------------------------
+// kernel dispatch tables
+// ------------------------------------------------------------------------------------------
+// instantiated in pass_inst.hip / grid_inst.hip (one object per storage type and block size)
+int launch_pass_1d_float_128_(int K, const PassArgs &a, int grid, int chunk, hipStream_t s);
+int launch_pass_1d_float_256_(int K, const PassArgs &a, int grid, int chunk, hipStream_t s);
+int launch_pass_1d_float_512_(int K, const PassArgs &a, int grid, int chunk, hipStream_t s);
+int launch_pass_1d_double_128_(int K, const PassArgs &a, int grid, int chunk, hipStream_t s);
+int launch_pass_1d_double_256_(int K, const PassArgs &a, int grid, int chunk, hipStream_t s);
+int launch_pass_1d_double_512_(int K, const PassArgs &a, int grid, int chunk, hipStream_t s);
+int launch_pass_grid_float(int K, const GridArgs &a, int grid, hipStream_t s);
+int launch_pass_grid_double(int K, const GridArgs &a, int grid, hipStream_t s);
+
+static int launch_pass_1d(bool f64, int K, const PassArgs &a, int grid, int block, int chunk,
+                          hipStream_t s) {
+    if (f64) {
+        if (block == 128) return launch_pass_1d_double_128_(K, a, grid, chunk, s);
+        if (block == 256) return launch_pass_1d_double_256_(K, a, grid, chunk, s);
+        if (block == 512) return launch_pass_1d_double_512_(K, a, grid, chunk, s);
+    } else {
+        if (block == 128) return launch_pass_1d_float_128_(K, a, grid, chunk, s);
+        if (block == 256) return launch_pass_1d_float_256_(K, a, grid, chunk, s);
+        if (block == 512) return launch_pass_1d_float_512_(K, a, grid, chunk, s);
+    }
+    return fail(PASC_ERR_ARG, "no fused pass for block %d", block);
+}
+
+// ------------------------------------------------------------------------------------------
+// allocation
+// ------------------------------------------------------------------------------------------
+// time steps per tile of the pass kernels: the FP64 phase of an FP32-bulk problem runs the
+// double instantiation with half the time steps per thread, so the tiling depends on the phase
+static int tile_len(const Engine &e, bool f64_phase) {
+    if (e.stencil != STENCIL_TIME) return 256;
+    return e.block * (f64_phase ? e.chunk64() : e.chunk);
+}
+
+static int engine_retile(Engine &e, int tl) {
+    e.cur_tile = tl;
+    int slots = 0;
+    for (auto &d : e.h_desc) {
+        d.tile0 = slots;
+        d.ntiles = (int32_t)((d.T + tl - 1) / tl);
+        slots += d.ntiles;
+    }
+    if (slots != e.total_slots || !e.d_part) {
+        if (e.d_part) (void)hipFree(e.d_part);
+        e.d_part = nullptr;
+        e.total_slots = slots;
+        HIP_TRY(hipMalloc(&e.d_part, sizeof(double) * (size_t)std::max(slots, 1) * NPART));
+    }
+    for (auto &g : e.groups) {
+        std::vector<int32_t> ids;
+        int64_t tiles = 0;
+        for (int i = 0; i < e.n_inst; i++)
+            if (e.h_desc[i].K == g.K) { ids.push_back(i); tiles += e.h_desc[i].ntiles; }
+        g.n = (int)ids.size();
+        g.max_tiles = tiles;
+        const int hl = (int)(tiles / 64 + 2);
+        if (hl > g.hint_len) {
+            if (g.d_hint) (void)hipFree(g.d_hint);
+            g.d_hint = nullptr;
+            HIP_TRY(hipMalloc(&g.d_hint, sizeof(int32_t) * hl));
+            g.hint_len = hl;
+        }
+        HIP_TRY(hipMemsetAsync(g.d_hint, 0, sizeof(int32_t) * g.hint_len, S_(e)));
+    }
+    HIP_TRY(hipMemcpyAsync(e.d_desc, e.h_desc.data(), sizeof(InstDesc) * e.n_inst,
+                           hipMemcpyHostToDevice, S_(e)));
+    HIP_TRY(hipStreamSynchronize(S_(e)));
+    return PASC_OK;
+}
+
+int engine_update_desc(Engine &e) {
+    HIP_TRY(hipMemcpyAsync(e.d_desc, e.h_desc.data(), sizeof(InstDesc) * e.n_inst,
+                           hipMemcpyHostToDevice, S_(e)));
+    HIP_TRY(hipStreamSynchronize(S_(e)));
+    return PASC_OK;
+}
+
+int engine_alloc(Engine &e, Ctx *ctx, int precision, int stencil, int32_t grid_width,
+                 const std::vector<InstDesc> &descs, int64_t data_elems) {
+    e.ctx = ctx;
+    e.precision = precision;
+    e.stencil = stencil;
+    e.grid_width = grid_width;
+    e.h_desc = descs;
+    e.n_inst = (int)descs.size();
+    e.data_elems = data_elems;
+    e.chunk = precision == PASC_FP32_BULK ? 8 : 4;
+    e.block = 256;
+    int64_t se = 0;
+    for (auto &d : descs) se = std::max(se, d.goff + (int64_t)d.K * d.ld);
+    e.state_elems = round_up(std::max<int64_t>(se, 64), 64);
+    const size_t es = precision == PASC_FP32_BULK ? 4 : 8;
+    for (int b = 0; b < 2; b++) {
+        HIP_TRY(hipMalloc(&e.d_gamma[b], es * (size_t)e.state_elems));
+        HIP_TRY(hipMemsetAsync(e.d_gamma[b], 0, es * (size_t)e.state_elems, S_(e)));
+        if (stencil == STENCIL_GRID) {
+            HIP_TRY(hipMalloc(&e.d_dir[b], es * (size_t)e.state_elems));
+            HIP_TRY(hipMemsetAsync(e.d_dir[b], 0, es * (size_t)e.state_elems, S_(e)));
+        }
+    }
+    const int64_t de = round_up(std::max<int64_t>(data_elems, 64), 64);
+    HIP_TRY(hipMalloc(&e.d_x64, 8 * (size_t)de));
+    HIP_TRY(hipMemsetAsync(e.d_x64, 0, 8 * (size_t)de, S_(e)));
+    if (precision == PASC_FP32_BULK) {
+        HIP_TRY(hipMalloc(&e.d_x32, 4 * (size_t)de));
+        HIP_TRY(hipMemsetAsync(e.d_x32, 0, 4 * (size_t)de, S_(e)));
+    }
+    HIP_TRY(hipMalloc(&e.d_desc, sizeof(InstDesc) * std::max(e.n_inst, 1)));
+    HIP_TRY(hipMalloc(&e.d_state, sizeof(InstState) * std::max(e.n_inst, 1)));
+    HIP_TRY(hipMalloc(&e.d_nactive, sizeof(int32_t)));
+    HIP_TRY(hipHostMalloc((void **)&e.h_nactive, sizeof(int32_t), hipHostMallocDefault));
+    e.h_parity.assign(e.n_inst, 0);
+    // K groups
+    for (int K = 1; K <= PASC_MAX_K; K++) {
+        std::vector<int32_t> ids;
+        for (int i = 0; i < e.n_inst; i++)
+            if (descs[i].K == K) ids.push_back(i);
+        if (ids.empty()) continue;
+        Engine::Group g;
+        g.K = K;
+        g.n = (int)ids.size();
+        HIP_TRY(hipMalloc(&g.d_ids, sizeof(int32_t) * ids.size()));
+        HIP_TRY(hipMemcpyAsync(g.d_ids, ids.data(), sizeof(int32_t) * ids.size(),
+                               hipMemcpyHostToDevice, S_(e)));
+        HIP_TRY(hipMalloc(&g.d_off, sizeof(int32_t) * (ids.size() + 1)));
+        e.groups.push_back(g);
+    }
+    if (e.groups.size() > (size_t)MAX_GROUPS) return fail(PASC_ERR_ARG, "too many K groups");
+    return engine_retile(e, tile_len(e, precision != PASC_FP32_BULK));
+}
+
+void engine_free(Engine &e) {
+    for (int b = 0; b < 2; b++) {
+        if (e.d_gamma[b]) (void)hipFree(e.d_gamma[b]);
+        if (e.d_gamma64[b]) (void)hipFree(e.d_gamma64[b]);
+        if (e.d_dir[b]) (void)hipFree(e.d_dir[b]);
+        if (e.d_dir64[b]) (void)hipFree(e.d_dir64[b]);
+        e.d_gamma[b] = e.d_gamma64[b] = e.d_dir[b] = e.d_dir64[b] = nullptr;
+    }
+    if (e.d_x64) (void)hipFree(e.d_x64);
+    if (e.d_x32) (void)hipFree(e.d_x32);
+    if (e.d_desc) (void)hipFree(e.d_desc);
+    if (e.d_state) (void)hipFree(e.d_state);
+    if (e.d_part) (void)hipFree(e.d_part);
+    if (e.d_nactive) (void)hipFree(e.d_nactive);
+    if (e.h_nactive) (void)hipHostFree(e.h_nactive);
+    if (e.d_scratch) (void)hipFree(e.d_scratch);
+    for (auto &g : e.groups) {
+        if (g.d_ids) (void)hipFree(g.d_ids);
+        if (g.d_off) (void)hipFree(g.d_off);
+        if (g.d_hint) (void)hipFree(g.d_hint);
+    }
+    e.groups.clear();
+    e.d_x64 = e.d_x32 = nullptr;
+    e.d_desc = nullptr;
+    e.d_state = nullptr;
+    e.d_part = nullptr;
+    e.d_nactive = nullptr;
+    e.h_nactive = nullptr;
+    e.d_scratch = nullptr;
+}
+
+static int scratch(Engine &e, size_t bytes, void **p) {
+    if (bytes > e.scratch_bytes) {
+        if (e.d_scratch) (void)hipFree(e.d_scratch);
+        e.d_scratch = nullptr;
+        e.scratch_bytes = 0;
+        HIP_TRY(hipMalloc(&e.d_scratch, bytes));
+        e.scratch_bytes = bytes;
+    }
+    *p = e.d_scratch;
+    return PASC_OK;
+}
+
+int engine_upload_data(Engine &e, const double *x_host, int64_t offset, int64_t n) {
+    HIP_TRY(hipMemcpyAsync((double *)e.d_x64 + offset, x_host, 8 * (size_t)n,
+                           hipMemcpyHostToDevice, S_(e)));
+    if (e.d_x32) {
+        const int64_t blocks = (n + 255) / 256;
+        hipLaunchKernelGGL((convert_kernel<double, float>), dim3((unsigned)blocks), dim3(256), 0,
+                           S_(e), (const double *)e.d_x64 + offset, (float *)e.d_x32 + offset, n);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipStreamSynchronize(S_(e)));
+    return PASC_OK;
+}
+
+// host copy of one instance's k-major rows (as double)
+static int download_rows(Engine &e, int inst, std::vector<double> &rows) {
+    const InstDesc &d = e.h_desc[inst];
+    const int par = e.h_parity[inst];
+    const size_t n = (size_t)d.K * d.ld;
+    rows.resize(n);
+    if (e.cur64) {
+        HIP_TRY(hipMemcpyAsync(rows.data(), (double *)e.d_gamma64[par] + d.goff, 8 * n,
+                               hipMemcpyDeviceToHost, S_(e)));
+    } else if (e.precision == PASC_FP32_BULK) {
+        std::vector<float> tmp(n);
+        HIP_TRY(hipMemcpyAsync(tmp.data(), (float *)e.d_gamma[par] + d.goff, 4 * n,
+                               hipMemcpyDeviceToHost, S_(e)));
+        HIP_TRY(hipStreamSynchronize(S_(e)));
+        for (size_t i = 0; i < n; i++) rows[i] = tmp[i];
+    } else {
+        HIP_TRY(hipMemcpyAsync(rows.data(), (double *)e.d_gamma[par] + d.goff, 8 * n,
+                               hipMemcpyDeviceToHost, S_(e)));
+    }
+    HIP_TRY(hipStreamSynchronize(S_(e)));
+    return PASC_OK;
+}
+
+static int upload_rows(Engine &e, int inst, const std::vector<double> &rows) {
+    const InstDesc &d = e.h_desc[inst];
+    const size_t n = (size_t)d.K * d.ld;
+    e.cur64 = false;
+    e.h_parity[inst] = 0;
+    if (e.precision == PASC_FP32_BULK) {
+        std::vector<float> tmp(n);
+        for (size_t i = 0; i < n; i++) tmp[i] = (float)rows[i];
+        HIP_TRY(hipMemcpyAsync((float *)e.d_gamma[0] + d.goff, tmp.data(), 4 * n,
+                               hipMemcpyHostToDevice, S_(e)));
+        HIP_TRY(hipStreamSynchronize(S_(e)));
+    } else {
+        HIP_TRY(hipMemcpyAsync((double *)e.d_gamma[0] + d.goff, rows.data(), 8 * n,
+                               hipMemcpyHostToDevice, S_(e)));
+        HIP_TRY(hipStreamSynchronize(S_(e)));
+    }
+    return PASC_OK;
+}
+
+int engine_set_uniform(Engine &e, int inst) {
+    const InstDesc &d = e.h_desc[inst];
+    std::vector<double> rows((size_t)d.K * d.ld, 0.0);
+    for (int k = 0; k < d.K; k++)
+        for (int64_t t = 0; t < d.T; t++) rows[k * d.ld + t] = 1.0 / d.K;
+    return upload_rows(e, inst, rows);
+}
+
+int engine_set_gamma(Engine &e, int inst, const double *g, int layout) {
+    const InstDesc &d = e.h_desc[inst];
+    std::vector<double> rows((size_t)d.K * d.ld, 0.0);
+    for (int k = 0; k < d.K; k++)
+        for (int64_t t = 0; t < d.T; t++)
+            rows[k * d.ld + t] = layout == PASC_LAYOUT_TMAJOR ? g[t * d.K + k] : g[k * d.T + t];
+    return upload_rows(e, inst, rows);
+}
+
+int engine_get_gamma(Engine &e, int inst, double *g, int layout) {
+    const InstDesc &d = e.h_desc[inst];
+    std::vector<double> rows;
+    int rc = download_rows(e, inst, rows);
+    if (rc) return rc;
+    for (int k = 0; k < d.K; k++)
+        for (int64_t t = 0; t < d.T; t++) {
+            if (layout == PASC_LAYOUT_TMAJOR) g[t * d.K + k] = rows[k * d.ld + t];
+            else g[k * d.T + t] = rows[k * d.ld + t];
+        }
+    return PASC_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// the iteration loop
+// ------------------------------------------------------------------------------------------
+struct PhaseCfg {
+    bool f64;
+    StepParams sp;
+};
+
+static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<int> &grids) {
+    hipStream_t s = S_(e);
+    void *g0 = ph.f64 && e.precision == PASC_FP32_BULK ? e.d_gamma64[0] : e.d_gamma[0];
+    void *g1 = ph.f64 && e.precision == PASC_FP32_BULK ? e.d_gamma64[1] : e.d_gamma[1];
+    const void *x = ph.f64 ? (const void *)e.d_x64 : (const void *)e.d_x32;
+    for (size_t gi = 0; gi < e.groups.size(); gi++) {
+        const auto &g = e.groups[gi];
+        const GroupView gv{g.d_ids, g.d_off, g.d_hint, g.n};
+        int rc;
+        if (e.stencil == STENCIL_TIME) {
+            PassArgs a{g0, g1, x, e.d_desc, e.d_state, e.d_part, gv};
+            rc = launch_pass_1d(ph.f64, g.K, a, grids[gi], e.block, ph.f64 ? e.chunk64() : e.chunk, s);
+        } else {
+            void *d0 = ph.f64 && e.precision == PASC_FP32_BULK ? e.d_dir64[0] : e.d_dir[0];
+            void *d1 = ph.f64 && e.precision == PASC_FP32_BULK ? e.d_dir64[1] : e.d_dir[1];
+            GridArgs a{g0, g1, d0, d1, x, e.d_desc, e.d_state, e.d_part, gv, e.grid_width};
+            rc = ph.f64 ? launch_pass_grid_double(g.K, a, grids[gi], s)
+                        : launch_pass_grid_float(g.K, a, grids[gi], s);
+        }
+        if (rc) return rc;
+    }
+    const int wpb = 256 / WAVE;
+    hipLaunchKernelGGL(spg_step, dim3((e.n_inst + wpb - 1) / wpb), dim3(256), 0, s, e.d_desc,
+                       e.d_state, e.d_part, e.n_inst, ph.sp);
+    ScanArgs sa{};
+    sa.n_groups = (int)e.groups.size();
+    for (size_t gi = 0; gi < e.groups.size(); gi++)
+        sa.g[gi] = ScanGroup{e.groups[gi].d_ids, e.groups[gi].d_off, e.groups[gi].d_hint, e.groups[gi].n};
+    sa.desc = e.d_desc;
+    sa.state = e.d_state;
+    sa.nactive = e.d_nactive;
+    hipLaunchKernelGGL(spg_scan, dim3(1), dim3(1024), 0, s, sa);
+    HIP_TRY(hipGetLastError());
+    return PASC_OK;
+}
+
+static int run_phase(Engine &e, const PhaseCfg &ph, const pasc_spg_params &p, int max_passes) {
+    hipStream_t s = S_(e);
+    const int tl = tile_len(e, ph.f64);
+    if (tl != e.cur_tile) {
+        int rc0 = engine_retile(e, tl);
+        if (rc0) return rc0;
+    }
+    // initial scan (active tiles of every group)
+    {
+        ScanArgs sa{};
+        sa.n_groups = (int)e.groups.size();
+        for (size_t gi = 0; gi < e.groups.size(); gi++)
+            sa.g[gi] = ScanGroup{e.groups[gi].d_ids, e.groups[gi].d_off, e.groups[gi].d_hint, e.groups[gi].n};
+        sa.desc = e.d_desc;
+        sa.state = e.d_state;
+        sa.nactive = e.d_nactive;
+        hipLaunchKernelGGL(spg_scan, dim3(1), dim3(1024), 0, s, sa);
+        HIP_TRY(hipGetLastError());
+    }
+    std::vector<int> grids;
+    for (auto &g : e.groups) {
+        int64_t cap = e.grid_cap > 0 ? e.grid_cap : (int64_t)std::max(e.ctx->n_cu, 1) * 32;
+        grids.push_back((int)std::max<int64_t>(1, std::min<int64_t>(g.max_tiles, cap)));
+    }
+    const int every = std::max(1, p.check_every);
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int passes = 0;
+    int rc = PASC_OK;
+    while (passes < max_passes) {
+        const int n = std::min(every, max_passes - passes);
+        if (p.use_graph && n == every) {
+            if (!exec) {
+                HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                for (int i = 0; i < n && rc == PASC_OK; i++) rc = enqueue_iteration(e, ph, grids);
+                hipError_t ec = hipStreamEndCapture(s, &graph);
+                if (rc) return rc;
+                if (ec != hipSuccess) return fail(PASC_ERR_HIP, "graph capture: %s", hipGetErrorString(ec));
+                ec = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                if (ec != hipSuccess) return fail(PASC_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ec));
+            }
+            HIP_TRY(hipGraphLaunch(exec, s));
+        } else {
+            for (int i = 0; i < n; i++) {
+                rc = enqueue_iteration(e, ph, grids);
+                if (rc) return rc;
+            }
+        }
+        passes += n;
+        HIP_TRY(hipMemcpyAsync(e.h_nactive, e.d_nactive, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (*e.h_nactive == 0) break;
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    return PASC_OK;
+}
+
+static void init_states(Engine &e, std::vector<InstState> &st, const pasc_spg_params &p,
+                        const std::vector<char> &mask, bool keep_alpha) {
+    for (int i = 0; i < e.n_inst; i++) {
+        InstState &s = st[i];
+        if (!mask[i]) { s.active = 0; continue; }
+        const double a = keep_alpha ? s.alpha_dir : p.alphainit;
+        const double nb = s.normb;
+        std::memset(&s, 0, sizeof(s));
+        s.normb = nb;
+        s.alpha_dir = (a > 0 && std::isfinite(a)) ? a : p.alphainit;
+        s.alpha_apply = s.alpha_dir;
+        s.best_kkt = std::numeric_limits<double>::infinity();
+        s.kkt = std::numeric_limits<double>::infinity();
+        s.first = 1;
+        s.active = 1;
+        s.parity = e.h_parity[i];
+        s.f = std::numeric_limits<double>::max();
+    }
+}
+
+static StepParams make_step(const pasc_spg_params &p, bool fp32_phase) {
+    StepParams sp{};
+    sp.maxit = p.maxit;
+    sp.m = std::min(std::max(p.m, 1), PASC_MAX_M);
+    sp.eps = p.eps;
+    sp.gamma = p.gamma;
+    sp.sigma1 = p.sigma1;
+    sp.sigma2 = p.sigma2;
+    sp.alphainit = p.alphainit;
+    sp.stop_mask = p.stop_mask;
+    sp.kkt_tol = p.kkt_tol;
+    sp.fp32_phase = fp32_phase ? 1 : 0;
+    sp.stall_window = std::max(1, p.stall_window);
+    sp.fp32_switch = p.fp32_switch;
+    return sp;
+}
+
+static int compute_normb(Engine &e, std::vector<InstState> &st) {
+    // ||b||_2 of every instance (only needed by the *_normb criteria)
+    for (int i = 0; i < e.n_inst; i++) {
+        double f, kkt, nb2;
+        int rc = engine_eval_raw(e, i, &f, &kkt, &nb2);
+        if (rc) return rc;
+        st[i].normb = std::sqrt(nb2);
+    }
+    return PASC_OK;
+}
+
+int engine_solve(Engine &e, const pasc_spg_params &p, pasc_spg_result *res_single,
+                 pasc_instance_result *res_multi, double *time_s) {
+    hipStream_t s = S_(e);
+    std::vector<InstState> st(e.n_inst);
+    std::memset(st.data(), 0, sizeof(InstState) * st.size());
+    if (e.cur64 && e.precision == PASC_FP32_BULK) {
+        // warm start (x0 aliases x, graphh1fem.cpp:136-137): the last iterate is FP64
+        for (int i = 0; i < e.n_inst; i++) {
+            const InstDesc &d = e.h_desc[i];
+            const int64_t n = (int64_t)d.K * d.ld;
+            hipLaunchKernelGGL((convert_kernel<double, float>), dim3((unsigned)((n + 255) / 256)),
+                               dim3(256), 0, s, (const double *)e.d_gamma64[e.h_parity[i]] + d.goff,
+                               (float *)e.d_gamma[0] + d.goff, n);
+            e.h_parity[i] = 0;
+        }
+        HIP_TRY(hipGetLastError());
+    }
+    e.cur64 = false;
+    if (p.stop_mask & (PASC_STOP_NORMGP_NORMB | PASC_STOP_ANORMGP_NORMB)) {
+        int rc = compute_normb(e, st);
+        if (rc) return rc;
+    }
+    std::vector<char> all(e.n_inst, 1);
+    init_states(e, st, p, all, false);
+    HIP_TRY(hipMemcpyAsync(e.d_state, st.data(), sizeof(InstState) * e.n_inst,
+                           hipMemcpyHostToDevice, s));
+    hipEvent_t ev0, ev1;
+    HIP_TRY(hipEventCreate(&ev0));
+    HIP_TRY(hipEventCreate(&ev1));
+    HIP_TRY(hipEventRecord(ev0, s));
+
+    const bool bulk32 = e.precision == PASC_FP32_BULK;
+    std::vector<int> it32(e.n_inst, 0), it64(e.n_inst, 0);
+    int rc;
+    if (!bulk32) {
+        PhaseCfg ph{true, make_step(p, false)};
+        rc = run_phase(e, ph, p, p.maxit + 2);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(st.data(), e.d_state, sizeof(InstState) * e.n_inst,
+                               hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (int i = 0; i < e.n_inst; i++) { it64[i] = st[i].it; e.h_parity[i] = st[i].parity; }
+    } else {
+        PhaseCfg ph{false, make_step(p, true)};
+        rc = run_phase(e, ph, p, p.maxit + 2);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(st.data(), e.d_state, sizeof(InstState) * e.n_inst,
+                               hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        // FP64 polish: promote the FP32 iterate and certify the target in FP64
+        if (!e.d_gamma64[0]) {
+            for (int b = 0; b < 2; b++) {
+                HIP_TRY(hipMalloc(&e.d_gamma64[b], 8 * (size_t)e.state_elems));
+                HIP_TRY(hipMemsetAsync(e.d_gamma64[b], 0, 8 * (size_t)e.state_elems, s));
+                if (e.stencil == STENCIL_GRID) {
+                    HIP_TRY(hipMalloc(&e.d_dir64[b], 8 * (size_t)e.state_elems));
+                    HIP_TRY(hipMemsetAsync(e.d_dir64[b], 0, 8 * (size_t)e.state_elems, s));
+                }
+            }
+        }
+        std::vector<char> pol(e.n_inst, 0);
+        for (int i = 0; i < e.n_inst; i++) {
+            it32[i] = st[i].it;
+            e.h_parity[i] = st[i].parity;
+            pol[i] = st[i].status == PASC_OK;
+            const InstDesc &d = e.h_desc[i];
+            const int64_t n = (int64_t)d.K * d.ld;
+            hipLaunchKernelGGL((convert_kernel<float, double>), dim3((unsigned)((n + 255) / 256)),
+                               dim3(256), 0, s, (const float *)e.d_gamma[st[i].parity] + d.goff,
+                               (double *)e.d_gamma64[0] + d.goff, n);
+            e.h_parity[i] = 0;
+        }
+        HIP_TRY(hipGetLastError());
+        init_states(e, st, p, pol, true);
+        HIP_TRY(hipMemcpyAsync(e.d_state, st.data(), sizeof(InstState) * e.n_inst,
+                               hipMemcpyHostToDevice, s));
+        pasc_spg_params pp = p;
+        pp.maxit = p.polish_maxit > 0 ? p.polish_maxit : p.maxit;
+        PhaseCfg ph2{true, make_step(pp, false)};
+        rc = run_phase(e, ph2, pp, pp.maxit + 2);
+        if (rc) return rc;
+        e.cur64 = true;
+        HIP_TRY(hipMemcpyAsync(st.data(), e.d_state, sizeof(InstState) * e.n_inst,
+                               hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (int i = 0; i < e.n_inst; i++) {
+            it64[i] = pol[i] ? st[i].it : 0;
+            e.h_parity[i] = st[i].parity;
+        }
+    }
+    HIP_TRY(hipEventRecord(ev1, s));
+    HIP_TRY(hipEventSynchronize(ev1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    if (time_s) *time_s = ms * 1e-3;
+    int worst = PASC_OK;
+    for (int i = 0; i < e.n_inst; i++) {
+        const InstState &x = st[i];
+        if (x.status) worst = x.status;
+        if (res_multi) {
+            pasc_instance_result &r = res_multi[i];
+            r.it = it32[i] + it64[i];
+            r.it_fp32 = it32[i];
+            r.it_fp64 = it64[i];
+            r.stop_reason = x.reason;
+            r.status = x.status;
+            r.fx = x.f;
+            r.kkt = x.kkt;
+        }
+    }
+    if (res_single && e.n_inst >= 1) {
+        const InstState &x = st[0];
+        pasc_spg_result &r = *res_single;
+        r.it = it32[0] + it64[0];
+        r.hessmult = r.it + 1;
+        r.status = x.status;
+        r.stop_reason = x.reason;
+        r.it_fp32 = it32[0];
+        r.it_fp64 = it64[0];
+        r.fx = x.f;
+        r.dd = x.dd;
+        r.dAd = x.dAd;
+        r.kkt = x.kkt;
+        r.normb = x.normb;
+        r.time_s = ms * 1e-3;
+    }
+    if (worst == PASC_ERR_NUMERIC) return fail(PASC_ERR_NUMERIC, "NaN/Inf in the SPG iteration");
+    return PASC_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// auxiliary computations on one instance (current iterate)
+// ------------------------------------------------------------------------------------------
+static const void *cur_gamma(Engine &e, int inst, bool &f32) {
+    const int par = e.h_parity[inst];
+    if (e.cur64) { f32 = false; return e.d_gamma64[par]; }
+    f32 = e.precision == PASC_FP32_BULK;
+    return e.d_gamma[par];
+}
+
+template <int K>
+static void launch_eval(Engine &e, const InstDesc &d, const void *g, bool f32, double *part,
+                        int blocks) {
+    hipStream_t s = S_(e);
+    if (f32)
+        hipLaunchKernelGGL((eval_kernel<float, K>), dim3(blocks), dim3(256), 0, s,
+                           (const float *)g + d.goff, (const double *)e.d_x64, d, e.stencil,
+                           (int64_t)e.grid_width, part);
+    else
+        hipLaunchKernelGGL((eval_kernel<double, K>), dim3(blocks), dim3(256), 0, s,
+                           (const double *)g + d.goff, (const double *)e.d_x64, d, e.stencil,
+                           (int64_t)e.grid_width, part);
+}
+
+int engine_eval_raw(Engine &e, int inst, double *fx, double *kkt, double *nb2) {
+    const InstDesc &d = e.h_desc[inst];
+    bool f32;
+    const void *g = cur_gamma(e, inst, f32);
+    const int blocks = (int)((d.T + 255) / 256);
+    void *part;
+    int rc = scratch(e, sizeof(double) * 3 * (size_t)blocks, &part);
+    if (rc) return rc;
+    switch (d.K) {
+        case 1: launch_eval<1>(e, d, g, f32, (double *)part, blocks); break;
+        case 2: launch_eval<2>(e, d, g, f32, (double *)part, blocks); break;
+        case 3: launch_eval<3>(e, d, g, f32, (double *)part, blocks); break;
+        case 4: launch_eval<4>(e, d, g, f32, (double *)part, blocks); break;
+        case 5: launch_eval<5>(e, d, g, f32, (double *)part, blocks); break;
+        case 6: launch_eval<6>(e, d, g, f32, (double *)part, blocks); break;
+        case 7: launch_eval<7>(e, d, g, f32, (double *)part, blocks); break;
+        case 8: launch_eval<8>(e, d, g, f32, (double *)part, blocks); break;
+        default: return fail(PASC_ERR_ARG, "unsupported K");
+    }
+    HIP_TRY(hipGetLastError());
+    std::vector<double> h(3 * (size_t)blocks);
+    HIP_TRY(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, S_(e)));
+    HIP_TRY(hipStreamSynchronize(S_(e)));
+    double f = 0, n2 = 0, k = 0;
+    for (int b = 0; b < blocks; b++) {
+        f += h[3 * b];
+        n2 += h[3 * b + 1];
+        const double m = h[3 * b + 2];
+        if (m > k || m != m) k = m;
+    }
+    if (fx) *fx = f;
+    if (kkt) *kkt = k;
+    if (nb2) *nb2 = n2;
+    return PASC_OK;
+}
+
+int engine_eval(Engine &e, int inst, double *fx, double *kkt) {
+    return engine_eval_raw(e, inst, fx, kkt, nullptr);
+}
+
+int engine_theta_update(Engine &e, int inst, double *theta_out) {
+    InstDesc &d = e.h_desc[inst];
+    bool f32;
+    const void *g = cur_gamma(e, inst, f32);
+    const int blocks = (int)((d.T + 255) / 256);
+    const int nv = d.K * (1 + d.xdim);
+    void *part;
+    int rc = scratch(e, sizeof(double) * (size_t)nv * blocks, &part);
+    if (rc) return rc;
+    if (f32)
+        hipLaunchKernelGGL((theta_partials<float>), dim3(blocks), dim3(256), 0, S_(e),
+                           (const float *)g + d.goff, (const double *)e.d_x64, d, (double *)part);
+    else
+        hipLaunchKernelGGL((theta_partials<double>), dim3(blocks), dim3(256), 0, S_(e),
+                           (const double *)g + d.goff, (const double *)e.d_x64, d, (double *)part);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> h((size_t)nv * blocks);
+    HIP_TRY(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, S_(e)));
+    HIP_TRY(hipStreamSynchronize(S_(e)));
+    std::vector<double> tot(nv, 0.0);
+    for (int b = 0; b < blocks; b++)
+        for (int v = 0; v < nv; v++) tot[v] += h[(size_t)b * nv + v];
+    // graphh1fem.cpp:374-379: theta_k = <x, gamma_k> / sum(gamma_k), 0 if the sum is 0
+    for (int k = 0; k < d.K; k++)
+        for (int c = 0; c < d.xdim; c++) {
+            const double gs = tot[k];
+            const double v = gs != 0 ? tot[d.K + k * d.xdim + c] / gs : 0.0;
+            d.theta[k * d.xdim + c] = v;
+            if (theta_out) theta_out[k * d.xdim + c] = v;
+        }
+    return engine_update_desc(e);
+}
+
+int engine_labels(Engine &e, int inst, int32_t *labels) {
+    const InstDesc &d = e.h_desc[inst];
+    bool f32;
+    const void *g = cur_gamma(e, inst, f32);
+    const int blocks = (int)((d.T + 255) / 256);
+    void *buf;
+    int rc = scratch(e, sizeof(int32_t) * (size_t)d.T, &buf);
+    if (rc) return rc;
+    if (f32)
+        hipLaunchKernelGGL((labels_kernel<float>), dim3(blocks), dim3(256), 0, S_(e),
+                           (const float *)g + d.goff, d, (int32_t *)buf);
+    else
+        hipLaunchKernelGGL((labels_kernel<double>), dim3(blocks), dim3(256), 0, S_(e),
+                           (const double *)g + d.goff, d, (int32_t *)buf);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(labels, buf, sizeof(int32_t) * d.T, hipMemcpyDeviceToHost, S_(e)));
+    HIP_TRY(hipStreamSynchronize(S_(e)));
+    return PASC_OK;
+}
+
+int engine_abserr(Engine &e, int inst, const double *solution, double *abserr) {
+    const InstDesc &d = e.h_desc[inst];
+    bool f32;
+    const void *g = cur_gamma(e, inst, f32);
+    const int blocks = (int)((d.T + 255) / 256);
+    void *buf;
+    const size_t need = sizeof(double) * ((size_t)d.T + blocks);
+    int rc = scratch(e, need, &buf);
+    if (rc) return rc;
+    double *sol = (double *)buf;
+    double *part = sol + d.T;
+    HIP_TRY(hipMemcpyAsync(sol, solution, sizeof(double) * d.T, hipMemcpyHostToDevice, S_(e)));
+    if (f32)
+        hipLaunchKernelGGL((abserr_kernel<float>), dim3(blocks), dim3(256), 0, S_(e),
+                           (const float *)g + d.goff, d, (const double *)sol, part);
+    else
+        hipLaunchKernelGGL((abserr_kernel<double>), dim3(blocks), dim3(256), 0, S_(e),
+                           (const double *)g + d.goff, d, (const double *)sol, part);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> h(blocks);
+    HIP_TRY(hipMemcpyAsync(h.data(), part, sizeof(double) * blocks, hipMemcpyDeviceToHost, S_(e)));
+    HIP_TRY(hipStreamSynchronize(S_(e)));
+    double a = 0;
+    for (double v : h) a += v;
+    *abserr = a;
+    return PASC_OK;
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void fill_uniform_kernel(const InstDesc *desc, S *g) {
+    const InstDesc &d = desc[blockIdx.y];
+    const int64_t n = (int64_t)d.K * d.ld;
+    const S v = S(1.0 / d.K);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = i % d.ld;
+        g[d.goff + i] = t < d.T ? v : S(0);
+    }
+}
+
+int engine_fill_uniform(Engine &e) {
+    e.cur64 = false;
+    for (int i = 0; i < e.n_inst; i++) e.h_parity[i] = 0;
+    const dim3 grid(64, (unsigned)e.n_inst);
+    if (e.precision == PASC_FP32_BULK)
+        hipLaunchKernelGGL((fill_uniform_kernel<float>), grid, dim3(256), 0, S_(e), e.d_desc, (float *)e.d_gamma[0]);
+    else
+        hipLaunchKernelGGL((fill_uniform_kernel<double>), grid, dim3(256), 0, S_(e), e.d_desc, (double *)e.d_gamma[0]);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(S_(e)));
+    return PASC_OK;
+}
+
+int engine_autotune(Engine &, int *, int *, double *) {
+    return fail(PASC_ERR_STATE, "autotune: not available in this build");
+}
+
+// ------------------------------------------------------------------------------------------
+// kernel-level entry points
+// ------------------------------------------------------------------------------------------
+template <typename S>
+static int project_dispatch(Ctx *ctx, S *x, int64_t nb, int K, int layout, int64_t ld) {
+    hipStream_t s = (hipStream_t)ctx->stream;
+    if (K <= 8) {
+        const unsigned blocks = (unsigned)((nb + 255) / 256);
+        switch (K) {
+#define PCASE(KK) case KK: hipLaunchKernelGGL((project_thread<S, S, KK>), dim3(blocks), dim3(256), 0, s, x, nb, layout, ld); break;
+            PCASE(1) PCASE(2) PCASE(3) PCASE(4) PCASE(5) PCASE(6) PCASE(7) PCASE(8)
+#undef PCASE
+        }
+    } else {
+        const unsigned blocks = (unsigned)((nb + 3) / 4);
+        hipLaunchKernelGGL((project_wave<S>), dim3(blocks), dim3(256), 0, s, x, nb, K, layout, ld);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    return PASC_OK;
+}
+
+int kernel_project(Ctx *ctx, void *x, int64_t nblocks, int K, int dtype, int layout, int64_t ld) {
+    if (nblocks == 0) return PASC_OK;
+    if (dtype == PASC_DTYPE_F32) return project_dispatch<float>(ctx, (float *)x, nblocks, K, layout, ld);
+    return project_dispatch<double>(ctx, (double *)x, nblocks, K, layout, ld);
+}
+
+int kernel_matvec(Ctx *ctx, const void *x, void *y, int64_t T, int64_t R, int K, int64_t ld,
+                  double alpha, int32_t grid_width, int dtype) {
+    hipStream_t s = (hipStream_t)ctx->stream;
+    const int64_t n = grid_width > 0 ? R : T;
+    if (n == 0) return PASC_OK;
+    const dim3 grid((unsigned)((n + 255) / 256), (unsigned)K);
+    if (grid_width > 0) {
+        if (dtype == PASC_DTYPE_F32)
+            hipLaunchKernelGGL((matvec_grid<float>), grid, dim3(256), 0, s, (const float *)x, (float *)y, R, K, ld, (int64_t)grid_width, alpha);
+        else
+            hipLaunchKernelGGL((matvec_grid<double>), grid, dim3(256), 0, s, (const double *)x, (double *)y, R, K, ld, (int64_t)grid_width, alpha);
+    } else {
+        if (dtype == PASC_DTYPE_F32)
+            hipLaunchKernelGGL((matvec_time<float>), grid, dim3(256), 0, s, (const float *)x, (float *)y, T, K, ld, alpha);
+        else
+            hipLaunchKernelGGL((matvec_time<double>), grid, dim3(256), 0, s, (const double *)x, (double *)y, T, K, ld, alpha);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    return PASC_OK;
+}
+
+}  // namespace pasc

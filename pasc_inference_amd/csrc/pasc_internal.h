@@ -1,0 +1,153 @@
+// Internal declarations shared by the HIP sources of libpasc_hip.so.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/pasc_hip.h"
+
+namespace pasc {
+
+// Sets the thread-local error message and returns code (printf-style message).
+int fail(int code, const char *fmt, ...);
+void clear_error();
+
+constexpr int NPART = 5;  // per-tile partial sums: dd, dAd, gd, f, kkt
+
+// Read-only description of one gamma QP instance on the device.
+struct InstDesc {
+    int64_t T;        // time steps (1-D) or nodes (grid graph, T == 1 in the reference sense)
+    int64_t ld;       // row stride of the k-major gamma rows (multiple of 64 elements)
+    int64_t goff;     // offset of row 0 in the gamma state buffers, in elements
+    int64_t xoff;     // offset of the instance's data in the data buffer, in elements
+    int32_t K;
+    int32_t tile0;    // first partial slot of this instance
+    int32_t ntiles;   // partial slots used per pass
+    int32_t xdim;
+    double alpha;     // A = alpha * A0
+    double bscale;    // b = -bscale * ||x - theta_k||^2
+    double s_kkt;     // step of the natural KKT residual
+    const uint8_t *heldout;  // device mask or nullptr
+    double theta[PASC_MAX_K * 3];  // theta[k*xdim + c], xdim <= 3
+};
+
+// SPG state of one instance, owned by the device step kernel.
+struct InstState {
+    double f, f_prev;
+    double dd_prev, dAd_prev;  // dots of the direction of the last completed iteration
+    double dd, dAd, gd;        // dots of the current direction
+    double alpha_dir;          // BB step defining the current direction d_i
+    double alpha_apply;        // BB step that defined d_{i-1} (recomputed in the next pass)
+    double beta_apply;         // beta_{i-1}, applied to d_{i-1} at the start of the next pass
+    double kkt, best_kkt;
+    double normb;
+    double fs[PASC_MAX_M];
+    int32_t fs_last;
+    int32_t it;                // completed reference iterations
+    int32_t first;             // next pass starts from an unprojected gamma_0
+    int32_t active;
+    int32_t reason;
+    int32_t status;
+    int32_t stall;
+    int32_t parity;            // state buffer holding the current iterate
+};
+
+// Solver constants copied to the step kernel.
+struct StepParams {
+    int32_t maxit;
+    int32_t m;
+    double eps;
+    double gamma, sigma1, sigma2, alphainit;
+    uint32_t stop_mask;
+    double kkt_tol;
+    int32_t fp32_phase;   // 1: FP32 phase (stagnation / switch rules apply)
+    int32_t stall_window;
+    double fp32_switch;
+};
+
+enum StencilKind { STENCIL_TIME = 0, STENCIL_GRID = 1 };
+
+struct Ctx;
+
+// Engine: one or many instances sharing state buffers (k-major rows, padded).
+struct Engine {
+    Ctx *ctx = nullptr;
+    int precision = PASC_FP64;  // storage type of the main state buffers
+    int stencil = STENCIL_TIME;
+    int32_t grid_width = 0;
+    int n_inst = 0;
+    int64_t state_elems = 0;     // elements per state buffer
+    int64_t data_elems = 0;
+    int total_slots = 0;
+    std::vector<InstDesc> h_desc;
+    std::vector<int> h_parity;   // buffer holding each instance's current iterate
+    bool cur64 = false;          // FP32-bulk: the current iterates live in the FP64 buffers
+    // device buffers
+    void *d_gamma[2] = {nullptr, nullptr};    // ping-pong state buffers (precision type)
+    void *d_gamma64[2] = {nullptr, nullptr};  // FP64 polish buffers (FP32-bulk only)
+    void *d_dir[2] = {nullptr, nullptr};      // grid stencil: direction ping-pong
+    void *d_dir64[2] = {nullptr, nullptr};
+    void *d_x64 = nullptr;                    // data, FP64, rows of stride ld
+    void *d_x32 = nullptr;                    // data, FP32 copy (FP32-bulk only)
+    InstDesc *d_desc = nullptr;
+    InstState *d_state = nullptr;
+    double *d_part = nullptr;                 // total_slots * NPART
+    void *d_scratch = nullptr;
+    size_t scratch_bytes = 0;
+    // K groups for the fused passes
+    struct Group {
+        int K = 0;
+        int n = 0;
+        int64_t max_tiles = 0;
+        int32_t *d_ids = nullptr;       // instance ids of the group
+        int32_t *d_off = nullptr;       // n+1 exclusive scan of active tile counts
+        int32_t *d_hint = nullptr;      // tile >> 6 -> first candidate position
+        int hint_len = 0;
+    };
+    std::vector<Group> groups;
+    int32_t *d_nactive = nullptr;        // device counter of active instances
+    int32_t *h_nactive = nullptr;        // pinned mirror
+    int block = 256;                     // threads per block of the fused 1-D pass
+    int chunk = 8;                       // time steps per thread (of the storage type)
+    int grid_cap = 0;                    // persistent grid cap (0: 32 blocks per CU)
+    int cur_tile = 0;                    // time steps per tile the partial slots are sized for
+    int chunk64() const { return precision == PASC_FP32_BULK ? chunk / 2 : chunk; }
+};
+
+struct Ctx {
+    int device = 0;
+    void *stream = nullptr;     // hipStream_t
+    bool own_stream = false;
+    int n_cu = 0;
+    std::string arch;
+    // autotune cache: key -> (block, chunk)
+    std::vector<std::pair<std::string, std::pair<int, int>>> tune;
+};
+
+// engine.hip
+int engine_alloc(Engine &e, Ctx *ctx, int precision, int stencil, int32_t grid_width,
+                 const std::vector<InstDesc> &descs, int64_t data_elems);
+void engine_free(Engine &e);
+int engine_upload_data(Engine &e, const double *x_host, int64_t offset, int64_t n);
+int engine_set_uniform(Engine &e, int inst);
+int engine_fill_uniform(Engine &e);
+int engine_set_gamma(Engine &e, int inst, const double *gamma_host, int layout);
+int engine_get_gamma(Engine &e, int inst, double *gamma_host, int layout);
+int engine_solve(Engine &e, const pasc_spg_params &p, pasc_spg_result *res_single,
+                 pasc_instance_result *res_multi, double *time_s);
+int engine_eval(Engine &e, int inst, double *fx, double *kkt);
+int engine_eval_raw(Engine &e, int inst, double *fx, double *kkt, double *nb2);
+int engine_update_desc(Engine &e);
+int engine_theta_update(Engine &e, int inst, double *theta_out);
+int engine_labels(Engine &e, int inst, int32_t *labels);
+int engine_abserr(Engine &e, int inst, const double *solution, double *abserr);
+int engine_adjoint(Engine &e, int inst, const uint8_t *mask, double cg_tol, int cg_maxit,
+                   double *loss, double *dlogeps, double *dtheta, int *cg_it);
+int engine_autotune(Engine &e, int *block, int *chunk, double *us);
+
+int kernel_project(Ctx *ctx, void *x, int64_t nblocks, int K, int dtype, int layout, int64_t ld);
+int kernel_matvec(Ctx *ctx, const void *x, void *y, int64_t T, int64_t R, int K, int64_t ld,
+                  double alpha, int32_t grid_width, int dtype);
+
+}  // namespace pasc

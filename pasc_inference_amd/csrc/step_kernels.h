@@ -1,0 +1,179 @@
+// Per-iteration scalar kernels of the SPG engine (included by engine.hip only).
+#pragma once
+
+#include "device_common.h"
+
+namespace pasc {
+
+// ---------------------------------------------------------------------------------------------
+// Step kernel: one wavefront per instance.  Reduces the instance's tile partials in a fixed
+// order (deterministic), then runs the scalar part of the reference iteration on lane 0:
+// spg_fs (spg_fs.cpp:17-63), the generalised Armijo step (spgqpsolver.cpp:182-195), the BB step
+// (:216) and the stopping tests (:260-274) plus the KKT test.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void spg_step(const InstDesc *desc, InstState *state,
+                                                 const double *part, int n_inst, StepParams P) {
+    const int w = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (w >= n_inst) return;
+    InstState &s = state[w];
+    if (!s.active) return;
+    const InstDesc &D = desc[w];
+    double v[NPART] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int i = lane; i < D.ntiles; i += WAVE) {
+        const double *p = part + (int64_t)(D.tile0 + i) * NPART;
+#pragma unroll
+        for (int q = 0; q < NPART - 1; q++) v[q] += p[q];
+        const double m = p[NPART - 1];
+        v[NPART - 1] = (m > v[NPART - 1] || m != m) ? m : v[NPART - 1];
+    }
+#pragma unroll
+    for (int q = 0; q < NPART - 1; q++) v[q] = wave_sum(v[q]);
+    v[NPART - 1] = wave_max(v[NPART - 1]);
+    if (lane != 0) return;
+
+    const double dd = v[0], dAd = v[1], gd = v[2], f = v[3], kkt = v[4];
+    s.parity ^= 1;  // the pass wrote gamma_i into the other buffer
+    s.kkt = kkt;
+    bool stop = false;
+    int reason = PASC_REASON_MAXIT;
+    if (!(f == f) || !(dd == dd) || !(dAd == dAd) || !(gd == gd) || !(kkt == kkt)) {
+        s.status = PASC_ERR_NUMERIC;
+        stop = true;
+        reason = PASC_REASON_NUMERIC;
+    } else if (s.first) {
+        for (int i = 0; i < P.m; i++) s.fs[i] = f;
+        s.fs_last = 0;
+        s.it = 0;
+        if ((P.stop_mask & PASC_STOP_KKT) && kkt <= P.kkt_tol) { stop = true; reason = PASC_REASON_KKT; }
+    } else {
+        s.fs_last = (s.fs_last + 1 >= P.m) ? 0 : s.fs_last + 1;
+        s.fs[s.fs_last] = f;
+        s.it += 1;
+        if ((P.stop_mask & PASC_STOP_DIFFF) && fabs(f - s.f) < P.eps) { stop = true; reason = PASC_REASON_DIFFF; }
+        else if ((P.stop_mask & PASC_STOP_NORMGP) && s.dd < P.eps) { stop = true; reason = PASC_REASON_NORMGP; }
+        else if ((P.stop_mask & PASC_STOP_NORMGP_NORMB) && s.dd < P.eps * s.normb) { stop = true; reason = PASC_REASON_NORMGP_NORMB; }
+        else if ((P.stop_mask & PASC_STOP_ANORMGP) && s.dAd < P.eps) { stop = true; reason = PASC_REASON_ANORMGP; }
+        else if ((P.stop_mask & PASC_STOP_ANORMGP_NORMB) && s.dAd < P.eps * s.normb) { stop = true; reason = PASC_REASON_ANORMGP_NORMB; }
+        else if ((P.stop_mask & PASC_STOP_KKT) && kkt <= P.kkt_tol) { stop = true; reason = PASC_REASON_KKT; }
+    }
+    if (!stop && s.it >= P.maxit) { stop = true; reason = PASC_REASON_MAXIT; }
+    if (!stop && dd == 0.0) { stop = true; reason = PASC_REASON_STATIONARY; }
+    if (!stop && P.fp32_phase) {
+        // FP32 phase ends at the switch threshold or when the KKT residual stagnates
+        if (kkt < 0.5 * s.best_kkt) { s.best_kkt = kkt; s.stall = 0; }
+        else s.stall += 1;
+        if ((P.fp32_switch > 0.0 && kkt <= P.fp32_switch) || s.stall >= P.stall_window) {
+            stop = true;
+            reason = PASC_REASON_MAXIT;
+        }
+    }
+    s.f_prev = s.f;
+    s.f = f;
+    s.dd = dd;
+    s.dAd = dAd;
+    s.gd = gd;
+    if (stop) {
+        s.active = 0;
+        s.reason = reason;
+        s.first = 0;
+        return;
+    }
+    double fmax = s.fs[0];
+    for (int i = 0; i < P.m; i++) fmax = s.fs[i] > fmax ? s.fs[i] : fmax;
+    double beta, alpha_next;
+    if (dAd > 0.0) {
+        const double xi = (fmax - f) / dAd;
+        const double beta_bar = -gd / dAd;
+        double beta_hat = P.gamma * beta_bar + sqrt(P.gamma * P.gamma * beta_bar * beta_bar + 2.0 * xi);
+        if (beta_hat < P.sigma1) beta_hat = P.sigma1;
+        beta = beta_hat < P.sigma2 ? beta_hat : P.sigma2;
+        alpha_next = dd / dAd;
+    } else {
+        // d lies in the null space of A (objective linear along d): the reference's IEEE
+        // arithmetic gives beta = sigma2 there; the BB step would be infinite, so restart it.
+        beta = P.sigma2;
+        alpha_next = P.alphainit;
+    }
+    s.dd_prev = dd;
+    s.dAd_prev = dAd;
+    s.beta_apply = beta;
+    s.alpha_apply = s.alpha_dir;
+    s.alpha_dir = alpha_next;
+    s.first = 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Active-tile scan: one workgroup rebuilds, for every K group, the exclusive scan of the tile
+// counts of its active instances and the tile -> position hint table, and counts the active
+// instances for the host poll.
+// ---------------------------------------------------------------------------------------------
+struct ScanGroup {
+    const int32_t *ids;
+    int32_t *off;
+    int32_t *hint;
+    int32_t n;
+};
+
+constexpr int MAX_GROUPS = PASC_MAX_K;
+
+struct ScanArgs {
+    ScanGroup g[MAX_GROUPS];
+    int32_t n_groups;
+    const InstDesc *desc;
+    const InstState *state;
+    int32_t *nactive;
+};
+
+__global__ __launch_bounds__(1024) void spg_scan(ScanArgs A) {
+    __shared__ int32_t sh[1024];
+    __shared__ int32_t carry;
+    int active_total = 0;
+    for (int gi = 0; gi < A.n_groups; gi++) {
+        const ScanGroup G = A.g[gi];
+        if (threadIdx.x == 0) carry = 0;
+        __syncthreads();
+        for (int base = 0; base < G.n; base += 1024) {
+            const int a = base + threadIdx.x;
+            int cnt = 0;
+            if (a < G.n) {
+                const int inst = G.ids[a];
+                cnt = A.state[inst].active ? A.desc[inst].ntiles : 0;
+                active_total += A.state[inst].active ? 1 : 0;
+            }
+            sh[threadIdx.x] = cnt;
+            __syncthreads();
+            for (int o = 1; o < 1024; o <<= 1) {
+                const int v = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+                __syncthreads();
+                sh[threadIdx.x] += v;
+                __syncthreads();
+            }
+            const int excl = carry + sh[threadIdx.x] - cnt;
+            if (a < G.n) G.off[a] = excl;
+            __syncthreads();
+            if (threadIdx.x == 1023) carry += sh[1023];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) G.off[G.n] = carry;
+        __syncthreads();
+        // hints: bucket b (tiles [64b, 64b+64)) starts at the position whose range holds 64b
+        for (int a = threadIdx.x; a < G.n; a += 1024) {
+            const int lo = G.off[a], hi = G.off[a + 1];
+            if (hi > lo) {
+                for (int b = (lo + 63) >> 6; (b << 6) < hi; b++) G.hint[b] = a;
+            }
+        }
+        __syncthreads();
+    }
+    __shared__ int32_t red[1024];
+    red[threadIdx.x] = active_total;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *A.nactive = red[0];
+}
+
+}  // namespace pasc

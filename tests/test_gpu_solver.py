@@ -1,0 +1,209 @@
+"""GPU parity of the gamma step, theta step and alternation against the FP64 CPU oracle.
+
+The SPG trajectory is path dependent (reduction order), so parity is checked on the converged
+minimiser (gamma max-abs, objective relative error, argmax labels) and, for the first
+iterations, on the objective sequence of the exact reference iteration.
+"""
+import numpy as np
+import pytest
+
+from helpers import grid_csr, label_agreement
+
+pytestmark = pytest.mark.gpu
+
+
+def _signal(seed, T, levels, p, sigma):
+    from pasc_inference_amd.solver import generate_markov
+    return generate_markov(seed, T, levels, p, sigma)
+
+
+def _oracle_min(oracle, x, theta, T, K, eps, tol=1e-11, graph=None, R=1, xdim=1, heldout=None):
+    b = oracle.assemble_b(x, theta, T, R, K, xdim=xdim, heldout=heldout)
+    alpha = eps / T
+    g0 = np.full(T * R * K, 1.0 / K)
+    xs, r = oracle.spg_solve(g0, b, T, R, K, alpha, graph=graph, maxit=200000, stop_difff=0,
+                             stop_kkt=1, kkt_tol=tol, kkt_scale=T, nthreads=8)
+    return xs, r, b
+
+
+@pytest.mark.parametrize("K,eps", [(2, 10.0), (3, 100.0), (4, 1.0)])
+def test_fp64_solve_matches_oracle_minimiser(ctx, oracle, K, eps):
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Problem
+    T = 20000
+    levels = [0.0, 1.0, 3.0, 4.5][:K]
+    x, _ = _signal(11 + K, T, levels, 1e-3, 0.8)
+    theta = np.array(levels)
+    ref, rr, b = _oracle_min(oracle, x, theta, T, K, eps)
+    print(f"oracle K={K} eps={eps}: it={rr.it} kkt={rr.kkt:.2e} f={rr.fx!r}", flush=True)
+    P = Problem(ctx, x, K, precision=capi.FP64, epssqr=eps)
+    P.set_theta(theta)
+    P.set_gamma_uniform()
+    r = P.gamma_solve(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=1e-10)
+    g = P.get_gamma()
+    f, kkt = P.objective()
+    print(f"gpu: it={r.it} kkt={r.kkt:.2e} f={r.fx!r} reason={capi.REASONS[r.stop_reason]}")
+    assert r.stop_reason == 6 and r.kkt <= 1e-10
+    assert abs(kkt - r.kkt) <= 1e-12 + 1e-3 * r.kkt
+    assert np.max(np.abs(g - ref)) <= 1e-6
+    assert abs(f - rr.fx) <= 1e-8 * abs(rr.fx)
+    assert abs(r.fx - f) <= 1e-10 * abs(f)
+    assert label_agreement(g, ref, K) >= 0.9999
+
+
+def test_fp64_iteration_semantics_follow_reference(ctx, oracle):
+    """Default reference settings (stop |df| < 1e-9): the device iteration reproduces the
+    oracle's objective sequence closely and stops at (nearly) the same iteration."""
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Problem
+    T, K, eps = 4000, 2, 10.0
+    x, _ = _signal(0, T, [1.0, 2.0], 0.002, 0.5)
+    theta = np.array([1.0, 2.0])
+    b = oracle.assemble_b(x, theta, T, 1, K)
+    for maxit in [1, 2, 5, 20]:
+        xs, rr = oracle.spg_solve(np.full(T * K, 0.5), b, T, 1, K, eps / T, maxit=maxit,
+                                  stop_difff=0)
+        P = Problem(ctx, x, K, precision=capi.FP64, epssqr=eps)
+        P.set_theta(theta)
+        P.set_gamma_uniform()
+        r = P.gamma_solve(maxit=maxit, stop_mask=0)
+        g = P.get_gamma()
+        print(f"maxit={maxit}: oracle f={rr.fx!r} gpu f={r.fx!r} dg={np.max(np.abs(g - xs)):.2e}")
+        assert r.it == maxit
+        assert abs(r.fx - rr.fx) <= 1e-12 * abs(rr.fx) + 1e-15
+        assert np.max(np.abs(g - xs)) <= 1e-10
+    xs, rr = oracle.spg_solve(np.full(T * K, 0.5), b, T, 1, K, eps / T)
+    P = Problem(ctx, x, K, precision=capi.FP64, epssqr=eps)
+    P.set_theta(theta)
+    P.set_gamma_uniform()
+    r = P.gamma_solve()
+    print(f"default stop: oracle it={rr.it} f={rr.fx!r}; gpu it={r.it} f={r.fx!r}")
+    assert abs(r.it - rr.it) <= max(3, rr.it // 10)
+    assert abs(r.fx - rr.fx) <= 1e-8 * abs(rr.fx)
+
+
+@pytest.mark.parametrize("eps", [1.0, 100.0])
+def test_fp32_bulk_reaches_fp64_certified_kkt(ctx, oracle, eps):
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Problem
+    T, K = 50000, 3
+    x, _ = _signal(1, T, [0.0, 1.0, 3.0], 1e-3, 1.0)
+    theta = np.array([0.0, 1.0, 3.0])
+    ref, rr, b = _oracle_min(oracle, x, theta, T, K, eps)
+    P = Problem(ctx, x, K, precision=capi.FP32_BULK, epssqr=eps)
+    P.set_theta(theta)
+    P.set_gamma_uniform()
+    r = P.gamma_solve(maxit=100000, stop_mask=capi.STOP_KKT, kkt_tol=1e-6)
+    g = P.get_gamma()
+    f, kkt = P.objective()
+    err = np.max(np.abs(g - ref))
+    print(f"fp32-bulk eps={eps}: it32={r.it_fp32} it64={r.it_fp64} kkt={kkt:.2e} "
+          f"gamma err={err:.2e} f rel={abs(f - rr.fx) / abs(rr.fx):.2e}")
+    assert kkt <= 1e-6
+    assert abs(f - rr.fx) <= 1e-8 * abs(rr.fx)
+    assert label_agreement(g, ref, K) >= 0.9999
+    assert err <= 1e-4
+
+
+def test_theta_update_matches_oracle(ctx, oracle):
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Problem
+    T, K = 30001, 3
+    x, _ = _signal(5, T, [0.0, 1.0, 3.0], 1e-3, 1.0)
+    rng = np.random.default_rng(3)
+    g = oracle.project_simplex(rng.random(T * K), K)
+    ref = oracle.theta_update(x, g, T, 1, K)
+    P = Problem(ctx, x, K, precision=capi.FP64)
+    P.set_gamma(g)
+    th = P.theta_update()
+    np.testing.assert_allclose(th, ref, rtol=1e-13)
+    Pm = Problem(ctx, x, K, precision=capi.FP32_BULK)
+    Pm.set_gamma(g)
+    th32 = Pm.theta_update()
+    np.testing.assert_allclose(th32, ref, rtol=1e-6)
+
+
+def test_alternation_c1_matches_oracle(ctx, oracle):
+    """Full FEM-H1 K-means alternation on the 1-D CPU-runnable configuration."""
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Problem, SIGNAL_C1 as C1
+    x, truth = _signal(C1["seed"], C1["T"], C1["levels"], C1["p_switch"], C1["sigma"])
+    T, K = C1["T"], 2
+    spg = dict(maxit=100000, stop_difff=0, stop_kkt=1, kkt_tol=1e-10, kkt_scale=T, nthreads=8)
+    g_o, th_o, tr_o = oracle.ts_solve(x, np.full(T * K, 0.5), C1["theta0"], T, 1, K, 1,
+                                      C1["eps"], ts=dict(maxit=50, eps=1e-10,
+                                                         skip_first_theta=1), **spg)
+    P = Problem(ctx, x, K, precision=capi.FP64, epssqr=C1["eps"])
+    P.set_theta(C1["theta0"])
+    P.set_gamma_uniform()
+    tr = P.ts_solve(ts=dict(maxit=50, eps=1e-10, skip_first_theta=1), maxit=100000,
+                    stop_mask=capi.STOP_KKT, kkt_tol=1e-10)
+    g = P.get_gamma()
+    th = P.get_theta()
+    print(f"oracle: it={tr_o.it} L={tr_o.L!r} theta={th_o}; gpu: it={tr.it} L={tr.L!r} theta={th}")
+    np.testing.assert_allclose(th, th_o, atol=1e-7)
+    assert np.max(np.abs(g - g_o)) <= 1e-6
+    assert abs(tr.L - tr_o.L) <= 1e-8 * abs(tr_o.L)
+    assert tr.increased == 0
+    assert label_agreement(g, g_o, K) >= 0.9999
+    assert np.mean(g.reshape(T, K).argmax(1) == truth) > 0.99
+
+
+def test_grid_image_matches_oracle(ctx, oracle):
+    """4-neighbour grid graph (T = 1), RGB data (xdim = 3, extension of the reference)."""
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Problem, generate_rect_image
+    w, h, K = 48, 40, 4
+    x, col, lab = generate_rect_image(9, w, h, K, 6, 0.15)
+    R = w * h
+    theta = col.copy()
+    eps = 3.0
+    graph = grid_csr(w, h)
+    b = oracle.assemble_b(x, theta, 1, R, K, xdim=3)
+    xs, rr = oracle.spg_solve(np.full(R * K, 1.0 / K), b, 1, R, K, eps, graph=graph,
+                              maxit=200000, stop_difff=0, stop_kkt=1, kkt_tol=1e-11,
+                              kkt_scale=1.0, nthreads=8)
+    for prec in [capi.FP64, capi.FP32_BULK]:
+        P = Problem(ctx, x, K, T=1, R=R, xdim=3, precision=prec, grid_width=w, epssqr=eps)
+        P.set_theta(theta)
+        P.set_gamma_uniform()
+        r = P.gamma_solve(maxit=200000, stop_mask=capi.STOP_KKT,
+                          kkt_tol=1e-10 if prec == capi.FP64 else 1e-7)
+        g = P.get_gamma()
+        f, kkt = P.objective()
+        print(f"grid prec={prec}: it={r.it} kkt={kkt:.2e} f={f!r} oracle f={rr.fx!r} "
+              f"err={np.max(np.abs(g - xs)):.2e}")
+        assert abs(f - rr.fx) <= 1e-8 * abs(rr.fx)
+        assert np.max(np.abs(g - xs)) <= (1e-6 if prec == capi.FP64 else 1e-4)
+        assert label_agreement(g, xs, K) >= 0.999
+    # theta update on the image (per-channel weighted means)
+    ref_th = oracle.theta_update(x, xs, 1, R, K, xdim=3)
+    np.testing.assert_allclose(P.theta_update(), ref_th, atol=1e-5)
+
+
+def test_batch_matches_individual_solves(ctx, oracle):
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Batch
+    rng = np.random.default_rng(2)
+    xs, Ks, ths = [], [], []
+    for i, (T, K) in enumerate([(3000, 2), (10000, 3), (777, 4), (5000, 3)]):
+        lv = np.sort(rng.uniform(0, 5, K))
+        x, _ = _signal(50 + i, T, lv, 5e-3, 0.6)
+        xs.append(x); Ks.append(K); ths.append(lv)
+    eps = np.array([0.1, 3.0, 100.0])
+    for prec in [capi.FP64, capi.FP32_BULK]:
+        B = Batch(ctx, xs, Ks, ths, eps, precision=prec)
+        res, t = B.solve(maxit=100000, stop_mask=capi.STOP_KKT, kkt_tol=1e-8 if prec == 0 else 1e-6)
+        for s in range(len(xs)):
+            T, K = xs[s].size, Ks[s]
+            for e in range(eps.size):
+                rr = res[s * eps.size + e]
+                g = B.get_gamma(s, e)
+                ref, ro, b = _oracle_min(oracle, xs[s], ths[s], T, K, eps[e])
+                fo = oracle.objective(g, b, T, 1, K, eps[e] / T)
+                print(f"prec={prec} s={s} e={e} it={rr.it} kkt={rr.kkt:.2e} "
+                      f"err={np.max(np.abs(g - ref)):.2e}")
+                assert rr.status == 0 and rr.stop_reason == 6
+                assert abs(fo - ro.fx) <= 1e-8 * abs(ro.fx)
+                assert label_agreement(g, ref, K) >= 0.999
+        B.close()

@@ -1,0 +1,272 @@
+"""Benchmark of the FEM-H1 K-means gamma step on one MI355X.
+
+Metric: time-to-KKT 1e-6 of one gamma QP at T=1e7, K=3 (levels {0,1,3}, switch 1e-4, noise
+sigma 1, seed 1, epsilon 100, theta fixed at the levels, gamma0 = 1/K), FP32-bulk path with FP64
+reductions and FP64-certified KKT; plus the batched epsilon-scan throughput (256 uneven series x
+16 epsilons) against the plain FP64 per-solve path on the same GPU.
+
+A "step" is one complete gamma solve from gamma0 to KKT <= 1e-6 with the data resident in HBM.
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--T", type=int, default=10_000_000)
+    ap.add_argument("--kkt", type=float, default=1e-6)
+    ap.add_argument("--no-scan", action="store_true")
+    ap.add_argument("--scan-series", type=int, default=256)
+    ap.add_argument("--plain-sample", type=int, default=48,
+                    help="instances of the scan timed on the plain FP64 per-solve path")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=4)
+    ap.add_argument("--verbose", action="store_true")
+    return ap.parse_args()
+
+
+def log(args, *a):
+    if args.verbose:
+        print(*a, file=sys.stderr, flush=True)
+
+
+def c2_workload(args):
+    from pasc_inference_amd.solver import SIGNAL_C2, generate_markov
+    c = SIGNAL_C2
+    x, lab = generate_markov(c["seed"], args.T, c["levels"], c["p_switch"], c["sigma"])
+    return x, lab, np.array(c["levels"]), c["eps"]
+
+
+def run_c2(args, ctx):
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Problem
+    x, truth, theta, eps = c2_workload(args)
+    T, K = args.T, theta.size
+    n = T * K
+    P32 = Problem(ctx, x, K, precision=capi.FP32_BULK, epssqr=eps)
+    P32.set_theta(theta)
+    params = dict(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt, check_every=32,
+                  use_graph=1)
+    for _ in range(args.warmup):
+        P32.set_gamma_uniform()
+        P32.gamma_solve(**params)
+    times, res = [], None
+    for _ in range(args.steps):
+        P32.set_gamma_uniform()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        res = P32.gamma_solve(**params)
+        ctx.synchronize()
+        times.append(time.perf_counter() - t0)
+        log(args, f"fp32-bulk solve: {times[-1]:.4f}s it={res.it_fp32}+{res.it_fp64} kkt={res.kkt:.2e}")
+    g32 = P32.get_gamma()
+    f32, kkt32 = P32.objective()
+    # one extra solve with hipEvents around every fused pass (kernel time for the roofline)
+    P32.set_profiling(True)
+    P32.set_gamma_uniform()
+    prof = P32.gamma_solve(**params)
+    P32.set_profiling(False)
+    P32.close()
+
+    # plain FP64 per-solve path (no graph, host poll every iteration)
+    P64 = Problem(ctx, x, K, precision=capi.FP64, epssqr=eps)
+    P64.set_theta(theta)
+    P64.set_gamma_uniform()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    r64 = P64.gamma_solve(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt)
+    ctx.synchronize()
+    t64 = time.perf_counter() - t0
+    log(args, f"plain fp64 solve: {t64:.4f}s it={r64.it} kkt={r64.kkt:.2e}")
+    # FP64 reference solve (tight KKT) for the error statement
+    P64.set_profiling(True)
+    P64.set_gamma_uniform()
+    rref = P64.gamma_solve(maxit=400000, stop_mask=capi.STOP_KKT, kkt_tol=1e-10, check_every=32,
+                           use_graph=0)
+    P64.set_profiling(False)
+    gref = P64.get_gamma()
+    fref, kref = P64.objective()
+    P64.close()
+    lab32 = g32.reshape(T, K).argmax(1)
+    labref = gref.reshape(T, K).argmax(1)
+    err = dict(
+        gamma_maxabs=float(np.max(np.abs(g32 - gref))),
+        f_rel=float(abs(f32 - fref) / abs(fref)),
+        label_agreement=float(np.mean(lab32 == labref)),
+        label_vs_truth=float(np.mean(lab32 == truth)),
+        kkt_fp64=float(kkt32),
+        ref_kkt=float(kref),
+        ref_it=int(rref.it),
+    )
+    # roofline of the dominant kernel: the fused FP32 pass
+    avg_pass = prof.pass_time_fp32_s / max(prof.passes_fp32, 1)
+    avg_pass64 = prof.pass_time_fp64_s / max(prof.passes_fp64, 1) if prof.passes_fp64 else None
+    alg_bytes = 6.0 * n * 4  # algorithmic bytes per SPG iteration, s = 4 (FP32)
+    own_bytes = 2.0 * n * 4 + T * 4  # what the fused pass must move: gamma in/out + data
+    achieved = alg_bytes / avg_pass / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_fused_pass_fp32_T1e7_K3.json")
+    if os.path.exists(tf):
+        try:
+            traffic = json.load(open(tf)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    c2 = dict(times=times, res=res, t64=t64, r64=r64, err=err, avg_pass=avg_pass,
+              avg_pass64=avg_pass64, achieved=achieved, own_bytes=own_bytes,
+              alg_bytes=alg_bytes, traffic=traffic, x=x, theta=theta, T=T, K=K,
+              ref_pass64=(rref.pass_time_fp64_s / max(rref.passes_fp64, 1)))
+    return c2
+
+
+def run_scan(args, ctx):
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import SCAN_EPS, Batch, Problem, generate_batch_params, \
+        generate_markov
+    T, K, lv, p, s, sd = generate_batch_params(2, args.scan_series)
+    xs, ths = [], []
+    for i in range(args.scan_series):
+        levels = lv[i, :K[i]]
+        x, _ = generate_markov(int(sd[i]), int(T[i]), levels, float(p[i]), float(s[i]))
+        xs.append(x)
+        ths.append(levels.copy())
+    eps = SCAN_EPS
+    n_inst = args.scan_series * eps.size
+    params = dict(maxit=400000, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt, check_every=64,
+                  use_graph=1)
+    B = Batch(ctx, xs, K, ths, eps, precision=capi.FP32_BULK)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    res, tdev = B.solve(**params)
+    ctx.synchronize()
+    tb = time.perf_counter() - t0
+    ok = sum(1 for r in res if r.status == 0 and r.kkt <= args.kkt)
+    its = np.array([r.it for r in res])
+    B.close()
+    log(args, f"batched scan: {tb:.3f}s for {n_inst} solves, {ok} certified, it max {its.max()}")
+    # plain FP64 per-solve path on a deterministic sample of the same instances
+    rng = np.random.default_rng(12345)
+    pick = rng.choice(n_inst, size=min(args.plain_sample, n_inst), replace=False)
+    tp = 0.0
+    plain_ok = 0
+    for j in pick:
+        si, ei = divmod(int(j), eps.size)
+        P = Problem(ctx, xs[si], int(K[si]), precision=capi.FP64, epssqr=float(eps[ei]))
+        P.set_theta(ths[si])
+        P.set_gamma_uniform()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        r = P.gamma_solve(maxit=400000, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt)
+        ctx.synchronize()
+        tp += time.perf_counter() - t0
+        plain_ok += int(r.status == 0 and r.kkt <= args.kkt)
+        P.close()
+    # extrapolate the sample to all instances by each instance's share of the batch work
+    plain_rate = len(pick) / tp
+    batched_rate = n_inst / tb
+    log(args, f"plain fp64 sample: {len(pick)} solves in {tp:.3f}s")
+    return dict(batched_solves_per_s=batched_rate, plain_solves_per_s=plain_rate,
+                ratio=batched_rate / plain_rate, batched_wall_s=tb, n_inst=n_inst,
+                certified=ok, plain_sample=len(pick), plain_certified=plain_ok,
+                it_max=int(its.max()), it_mean=float(its.mean()))
+
+
+def run_cpu_baseline(args, c2):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    x, theta, T, K = c2["x"], c2["theta"], c2["T"], c2["K"]
+    b = pyoracle.assemble_b(x, theta, T, 1, K)
+    g0 = np.full(T * K, 1.0 / K)
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    cores = max(1, min(cores, os.cpu_count() or 1))
+    out = {}
+    for nth in sorted({1, cores}):
+        t0 = time.perf_counter()
+        pyoracle.spg_solve(g0, b, T, 1, K, 100.0 / T, maxit=args.cpu_iters, stop_difff=0,
+                           nthreads=nth)
+        dt = time.perf_counter() - t0
+        out[nth] = dt / args.cpu_iters
+    it_needed = c2["r64"].it
+    best = min(out, key=lambda k: out[k])
+    return dict(
+        value=out[best] * it_needed, unit="s", cores=best, kind="port",
+        sample=(f"{args.cpu_iters} op-by-op FP64 SPG iterations of the T={T} K={K} solve on "
+                f"{best} host thread(s) ({out[best] * 1e3:.1f} ms/iteration; single thread "
+                f"{out[1] * 1e3:.1f} ms/iteration) x {it_needed} iterations to KKT "
+                f"{args.kkt:g} (the FP64 device path's count for the same algorithm)"),
+        ms_per_iteration={str(k): v * 1e3 for k, v in out.items()},
+    )
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path "
+              f"(requested --gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+    from pasc_inference_amd.solver import Context
+    with Context(0) as ctx:
+        c2 = run_c2(args, ctx)
+        scan = None if args.no_scan else run_scan(args, ctx)
+    cpu = None if args.no_cpu else run_cpu_baseline(args, c2)
+    ms = float(np.mean(c2["times"])) * 1e3
+    r = c2["res"]
+    line = {
+        "metric": "time-to-KKT 1e-6 at T=1e7 K=3; batched eps-scan solves/s vs plain, same GPU",
+        "value": float(np.mean(c2["times"])),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32 bulk + f64 reductions/polish",
+        "data": "synthetic (deterministic SplitMix64 Markov-chain signal generated on the host)",
+        "config": {"workload": f"gamma QP solve, T={c2['T']}, K={c2['K']}, levels {{0,1,3}}, "
+                               f"switch 1e-4, sigma 1, seed 1, epsilon 100, theta fixed, "
+                               f"gamma0=1/K, KKT<={args.kkt:g}",
+                   "T": c2["T"], "K": c2["K"], "epssqr": 100.0, "kkt_tol": args.kkt},
+        "iterations": {"fp32": r.it_fp32, "fp64_polish": r.it_fp64,
+                       "plain_fp64": c2["r64"].it},
+        "plain_fp64_time_to_kkt_s": c2["t64"],
+        "speedup_vs_plain_fp64": c2["t64"] / float(np.mean(c2["times"])),
+        "errors_vs_fp64_reference_solve": c2["err"],
+        "roofline": {
+            "bound": "hbm",
+            "achieved": c2["achieved"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": c2["achieved"] / HBM_PEAK_GBS,
+            "traffic": c2["traffic"],
+            "kernel": "spg_pass_1d<float,K=3> (fused SPG iteration)",
+            "avg_launch_us": c2["avg_pass"] * 1e6,
+            "algorithmic_bytes_per_launch": c2["alg_bytes"],
+            "own_min_bytes_per_launch": c2["own_bytes"],
+            "own_min_frac": c2["own_bytes"] / c2["avg_pass"] / 1e9 / HBM_PEAK_GBS,
+            "fp64_pass_avg_us": (c2["ref_pass64"] * 1e6) if c2["ref_pass64"] else None,
+        },
+        "scan": scan,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

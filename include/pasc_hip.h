@@ -116,6 +116,10 @@ typedef struct {
     double kkt;           /* natural KKT residual of the returned gamma (FP64 arithmetic) */
     double normb;         /* ||b||_2 */
     double time_s;        /* wall time of the solve on the device (hipEvent) */
+    double pass_time_fp32_s; /* with profiling on: summed device time of the fused SPG pass */
+    double pass_time_fp64_s; /*   kernels of each phase (hipEvents around every launch) */
+    int32_t passes_fp32;     /* fused passes launched in each phase */
+    int32_t passes_fp64;
 } pasc_spg_result;
 
 /* Outer alternation settings (tssolver.h:13-15) */
@@ -233,6 +237,10 @@ int pasc_batch_get_gamma(pasc_batch *b, int32_t series, int32_t eps_index, doubl
  * mask is [T*R] (1 = held out).  dtheta has K entries (xdim = 1). */
 int pasc_adjoint(pasc_problem *p, const uint8_t *mask, double cg_tol, int32_t cg_maxit,
                  double *loss, double *dlogeps, double *dtheta, int32_t *cg_it);
+
+/* ---- profiling: record hipEvents around every fused-pass launch of later solves (graph
+ * capture is disabled meanwhile); the totals are returned in pasc_spg_result. ---- */
+int pasc_set_profiling(pasc_problem *p, int enable);
 
 /* ---- autotuner ----
  * Times the fused SPG pass for the problem's (K, precision, T) over the candidate

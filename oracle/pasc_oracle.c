@@ -250,6 +250,18 @@ static double v_dot(const double *a, const double *b, int64_t n, int nthreads) {
     return s;
 }
 
+/* sum_t sum_k (g_kt - g_0t) d_kt: the value of dot(g, d) when every block of d sums to 0 */
+static double dot_centered(const double *d, const double *g, int64_t nblocks, int K,
+                           int nthreads) {
+    double s = 0.0;
+#pragma omp parallel for schedule(static) num_threads(nthreads) reduction(+ : s)
+    for (int64_t t = 0; t < nblocks; t++) {
+        const double c = g[t * K];
+        for (int k = 1; k < K; k++) s += (g[t * K + k] - c) * d[t * K + k];
+    }
+    return s;
+}
+
 /* spg_fs.cpp:17-63 */
 typedef struct {
     int m;
@@ -349,7 +361,7 @@ int or_spg_solve(double *x, const double *b, int64_t T, int64_t R, int K, double
         /* compute_dots :348-367 */
         dd = v_dot(d, d, n, nthreads);
         dAd = v_dot(d, Ad, n, nthreads);
-        gd = v_dot(d, g, n, nthreads);
+        gd = p->gd_centered ? dot_centered(d, g, nblocks, K, nthreads) : v_dot(d, g, n, nthreads);
 
         fx_max = fs_get_max(&fs); /* :176 */
 
@@ -369,7 +381,10 @@ int or_spg_solve(double *x, const double *b, int64_t T, int64_t R, int K, double
 
         /* :207-211 */
         fx_old = fx;
-        fx = spg_get_fx(x, g, b, temp, n, nthreads);
+        if (p->fx_identity)
+            fx = fx_old + beta * (gd + 0.5 * beta * dAd); /* spgqpsolver.h:835-842 */
+        else
+            fx = spg_get_fx(x, g, b, temp, n, nthreads);
         fs_update(&fs, fx);
 
         /* :216 */

@@ -76,6 +76,9 @@ typedef struct {
     double kkt_tol;
     double kkt_scale;     /* s of or_kkt_residual */
     int nthreads;
+    int fx_identity;      /* f_new = f + beta*(gd + beta*dAd/2), spgqpsolver.h:835-842 */
+    int gd_centered;      /* gd = sum_t sum_k (g_kt - g_0t) d_kt (equal in exact arithmetic,
+                             since sum_k d_kt = 0; removes the common-mode rounding) */
 } or_spg_params;
 
 typedef struct {

@@ -55,7 +55,8 @@ class SpgResult(C.Structure):
         ("it", C.c_int32), ("hessmult", C.c_int32), ("status", C.c_int32),
         ("stop_reason", C.c_int32), ("it_fp32", C.c_int32), ("it_fp64", C.c_int32),
         ("fx", C.c_double), ("dd", C.c_double), ("dAd", C.c_double), ("kkt", C.c_double),
-        ("normb", C.c_double), ("time_s", C.c_double),
+        ("normb", C.c_double), ("time_s", C.c_double), ("pass_time_fp32_s", C.c_double),
+        ("pass_time_fp64_s", C.c_double), ("passes_fp32", C.c_int32), ("passes_fp64", C.c_int32),
     ]
 
 
@@ -119,6 +120,7 @@ _SIGS = {
     "pasc_adjoint": ([vp, C.POINTER(C.c_uint8), C.c_double, C.c_int32, C.POINTER(C.c_double),
                       C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int32)],
                      C.c_int),
+    "pasc_set_profiling": ([vp, C.c_int], C.c_int),
     "pasc_autotune": ([vp, C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                        C.POINTER(C.c_double)], C.c_int),
     "pasc_generate_markov": ([C.c_uint64, C.c_int64, C.c_int32, C.POINTER(C.c_double),

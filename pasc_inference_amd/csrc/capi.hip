@@ -558,6 +558,12 @@ int pasc_adjoint(pasc_problem *p, const uint8_t *mask, double cg_tol, int32_t cg
     return rc;
 }
 
+int pasc_set_profiling(pasc_problem *p, int enable) {
+    if (!p) return fail(PASC_ERR_ARG, "set_profiling: NULL problem");
+    p->e.profile = enable != 0;
+    return PASC_OK;
+}
+
 int pasc_autotune(pasc_problem *p, const char *cache_path, int32_t *block, int32_t *chunk,
                   double *us_per_pass) {
     clear_error();

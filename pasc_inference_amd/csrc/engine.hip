@@ -299,8 +299,15 @@ struct PhaseCfg {
     StepParams sp;
 };
 
-static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<int> &grids) {
+static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<int> &grids,
+                             std::vector<hipEvent_t> *ev = nullptr) {
     hipStream_t s = S_(e);
+    if (ev) {
+        hipEvent_t a;
+        HIP_TRY(hipEventCreate(&a));
+        HIP_TRY(hipEventRecord(a, s));
+        ev->push_back(a);
+    }
     void *g0 = ph.f64 && e.precision == PASC_FP32_BULK ? e.d_gamma64[0] : e.d_gamma[0];
     void *g1 = ph.f64 && e.precision == PASC_FP32_BULK ? e.d_gamma64[1] : e.d_gamma[1];
     const void *x = ph.f64 ? (const void *)e.d_x64 : (const void *)e.d_x32;
@@ -319,6 +326,12 @@ static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<in
                         : launch_pass_grid_float(g.K, a, grids[gi], s);
         }
         if (rc) return rc;
+    }
+    if (ev) {
+        hipEvent_t b;
+        HIP_TRY(hipEventCreate(&b));
+        HIP_TRY(hipEventRecord(b, s));
+        ev->push_back(b);
     }
     const int wpb = 256 / WAVE;
     hipLaunchKernelGGL(spg_step, dim3((e.n_inst + wpb - 1) / wpb), dim3(256), 0, s, e.d_desc,
@@ -362,11 +375,12 @@ static int run_phase(Engine &e, const PhaseCfg &ph, const pasc_spg_params &p, in
     const int every = std::max(1, p.check_every);
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
+    std::vector<hipEvent_t> evs;
     int passes = 0;
     int rc = PASC_OK;
     while (passes < max_passes) {
         const int n = std::min(every, max_passes - passes);
-        if (p.use_graph && n == every) {
+        if (p.use_graph && n == every && !e.profile) {
             if (!exec) {
                 HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
                 for (int i = 0; i < n && rc == PASC_OK; i++) rc = enqueue_iteration(e, ph, grids);
@@ -379,7 +393,7 @@ static int run_phase(Engine &e, const PhaseCfg &ph, const pasc_spg_params &p, in
             HIP_TRY(hipGraphLaunch(exec, s));
         } else {
             for (int i = 0; i < n; i++) {
-                rc = enqueue_iteration(e, ph, grids);
+                rc = enqueue_iteration(e, ph, grids, e.profile ? &evs : nullptr);
                 if (rc) return rc;
             }
         }
@@ -390,6 +404,18 @@ static int run_phase(Engine &e, const PhaseCfg &ph, const pasc_spg_params &p, in
     }
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (!evs.empty()) {
+        HIP_TRY(hipStreamSynchronize(s));
+        double ms = 0.0;
+        for (size_t i = 0; i + 1 < evs.size(); i += 2) {
+            float m = 0.f;
+            HIP_TRY(hipEventElapsedTime(&m, evs[i], evs[i + 1]));
+            ms += m;
+        }
+        for (auto ev : evs) (void)hipEventDestroy(ev);
+        e.prof_ms[ph.f64 ? 1 : 0] += ms;
+        e.prof_n[ph.f64 ? 1 : 0] += (int)(evs.size() / 2);
+    }
     return PASC_OK;
 }
 
@@ -459,6 +485,8 @@ int engine_solve(Engine &e, const pasc_spg_params &p, pasc_spg_result *res_singl
         HIP_TRY(hipGetLastError());
     }
     e.cur64 = false;
+    e.prof_ms[0] = e.prof_ms[1] = 0.0;
+    e.prof_n[0] = e.prof_n[1] = 0;
     if (p.stop_mask & (PASC_STOP_NORMGP_NORMB | PASC_STOP_ANORMGP_NORMB)) {
         int rc = compute_normb(e, st);
         if (rc) return rc;
@@ -568,6 +596,10 @@ int engine_solve(Engine &e, const pasc_spg_params &p, pasc_spg_result *res_singl
         r.kkt = x.kkt;
         r.normb = x.normb;
         r.time_s = ms * 1e-3;
+        r.pass_time_fp32_s = e.prof_ms[0] * 1e-3;
+        r.pass_time_fp64_s = e.prof_ms[1] * 1e-3;
+        r.passes_fp32 = e.prof_n[0];
+        r.passes_fp64 = e.prof_n[1];
     }
     if (worst == PASC_ERR_NUMERIC) return fail(PASC_ERR_NUMERIC, "NaN/Inf in the SPG iteration");
     return PASC_OK;

@@ -112,6 +112,9 @@ struct Engine {
     int chunk = 8;                       // time steps per thread (of the storage type)
     int grid_cap = 0;                    // persistent grid cap (0: 32 blocks per CU)
     int cur_tile = 0;                    // time steps per tile the partial slots are sized for
+    bool profile = false;                // hipEvents around every pass launch
+    double prof_ms[2] = {0.0, 0.0};      // summed pass time per phase (FP32, FP64)
+    int prof_n[2] = {0, 0};              // passes launched per phase
     int chunk64() const { return precision == PASC_FP32_BULK ? chunk / 2 : chunk; }
 };
 

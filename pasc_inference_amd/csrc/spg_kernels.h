@@ -154,6 +154,10 @@ __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int ch
             const int64_t t = t0 - 4 + j;
             if (t < T) {
                 C z[K];
+                // <g, d> with the per-step common mode g_0t removed: every block of d sums to 0,
+                // so the value is unchanged, but the rounding of sum_k d_kt no longer multiplies
+                // the large data term (which otherwise stalls SPG near 1e-9 KKT).
+                const C gc = Gi[0][j - 4];
 #pragma unroll
                 for (int k = 0; k < K; k++) {
                     const C d = Dd[k][j];
@@ -163,7 +167,7 @@ __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int ch
                     const C q = obs[j] ? bscale * dx * dx : C(0);
                     acc[0] += (double)d * (double)d;
                     acc[1] += (double)d * (double)Ad;
-                    acc[2] += (double)g * (double)d;
+                    acc[2] += ((double)g - (double)gc) * (double)d;
                     acc[3] += 0.5 * ((double)g + (double)q) * (double)G1[k][j];
                     z[k] = G1[k][j] - s_kkt * g;
                 }
@@ -322,7 +326,7 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_dir(GridArgs A) {
                 const C d = y[k] - g0[k];
                 dst[k * D.ld + r] = S(g0[k]);
                 dr[k * D.ld + r] = S(d);
-                acc[2] += (double)g[k] * (double)d;
+                acc[2] += ((double)g[k] - (double)g[0]) * (double)d;  // common mode removed
                 acc[3] += 0.5 * ((double)g[k] + (double)q[k]) * (double)g0[k];
                 const double e = fabs((double)g0[k] - (double)z[k]);
                 acc[4] = (e > acc[4] || e != e) ? e : acc[4];

@@ -144,6 +144,9 @@ class Problem:
                                  dptr(dth), C.byref(it)))
         return loss.value, dle.value, dth, it.value
 
+    def set_profiling(self, enable: bool = True):
+        check(lib().pasc_set_profiling(self.h, int(enable)))
+
     def autotune(self, cache_path: str | None = None):
         b, c, us = C.c_int32(), C.c_int32(), C.c_double()
         check(lib().pasc_autotune(self.h, cache_path.encode() if cache_path else None,

@@ -97,8 +97,9 @@ def run_c2(args, ctx):
     # FP64 reference solve (tight KKT) for the error statement
     P64.set_profiling(True)
     P64.set_gamma_uniform()
-    rref = P64.gamma_solve(maxit=400000, stop_mask=capi.STOP_KKT, kkt_tol=1e-10, check_every=32,
+    rref = P64.gamma_solve(maxit=50000, stop_mask=capi.STOP_KKT, kkt_tol=1e-10, check_every=32,
                            use_graph=0)
+    log(args, f"fp64 reference solve: it={rref.it} kkt={rref.kkt:.2e}")
     P64.set_profiling(False)
     gref = P64.get_gamma()
     fref, kref = P64.objective()
@@ -176,6 +177,8 @@ def run_scan(args, ctx):
         tp += time.perf_counter() - t0
         plain_ok += int(r.status == 0 and r.kkt <= args.kkt)
         P.close()
+        log(args, f"plain fp64 instance {j}: T={xs[si].size} K={K[si]} eps={eps[ei]:.3g} "
+                  f"it={r.it} kkt={r.kkt:.2e}")
     # extrapolate the sample to all instances by each instance's share of the batch work
     plain_rate = len(pick) / tp
     batched_rate = n_inst / tb

@@ -98,8 +98,8 @@ typedef struct {
     int32_t use_graph;    /* capture check_every iterations in a hipGraph (default 0) */
     double fp32_switch;   /* FP32-bulk: hand over to FP64 once the FP32 KKT <= this
                              (default 0 = decide by stagnation only) */
-    int32_t stall_window; /* FP32-bulk: iterations without a 2x KKT improvement that end the
-                             FP32 phase (default 64) */
+    int32_t stall_window; /* FP32-bulk: iterations without a 10% improvement of the best KKT
+                             residual that end the FP32 phase (default 100) */
     int32_t polish_maxit; /* FP32-bulk: maximum FP64 polish iterations (default maxit) */
 } pasc_spg_params;
 

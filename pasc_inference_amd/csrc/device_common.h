@@ -42,8 +42,18 @@ template <typename C> __device__ inline C cmax(C a, C b) { return a > b ? a : b;
 //   * the exact threshold is then the fixed point tau = (sum_{y > tau} y - 1) / |{y > tau}|
 //     iterated from the lower bracket end: the support only shrinks and at most K sweeps are
 //     needed, so the result is the same threshold as Chen & Ye's sorted search.
+template <typename C, int K>
+__device__ __forceinline__ C recip_small(int n) {
+    // 1/n for n in [1, K] without a division (select chain over compile-time constants)
+    C r = C(1);
+#pragma unroll
+    for (int m = 2; m <= K; m++) r = (n == m) ? C(1) / C(m) : r;
+    return r;
+}
+
+// Returns the threshold tau (0 when the input is already feasible and left unchanged).
 template <int K, typename C>
-__device__ __forceinline__ void project_simplex(C (&y)[K]) {
+__device__ __forceinline__ C project_simplex(C (&y)[K]) {
     C s = C(0);
     bool neg = false;
 #pragma unroll
@@ -51,7 +61,7 @@ __device__ __forceinline__ void project_simplex(C (&y)[K]) {
         s += y[k];
         neg |= (y[k] < C(0));
     }
-    if (!neg && s == C(1)) return;
+    if (!neg && s == C(1)) return C(0);
     C mx = y[0];
 #pragma unroll
     for (int k = 1; k < K; k++) mx = cmax(mx, y[k]);
@@ -81,10 +91,11 @@ __device__ __forceinline__ void project_simplex(C (&y)[K]) {
         }
         if (cnt == cnt_prev || cnt == 0) break;
         cnt_prev = cnt;
-        tau = (sum - C(1)) / C(cnt);
+        tau = (sum - C(1)) * recip_small<C, K>(cnt);
     }
 #pragma unroll
     for (int k = 0; k < K; k++) y[k] = cmax(y[k] - tau, C(0));
+    return tau;
 }
 
 // Wavefront reductions (64 lanes).

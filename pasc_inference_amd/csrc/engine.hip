@@ -63,7 +63,9 @@ static int launch_pass_1d(bool f64, int K, const PassArgs &a, int grid, int bloc
 // double instantiation with half the time steps per thread, so the tiling depends on the phase
 static int tile_len(const Engine &e, bool f64_phase) {
     if (e.stencil != STENCIL_TIME) return 256;
-    return e.block * (f64_phase ? e.chunk64() : e.chunk);
+    const int ch = f64_phase ? e.chunk64() : e.chunk;
+    const int ht = (3 + ch - 1) / ch;  // halo threads per side (see tile_steps)
+    return (e.block - 2 * ht) * ch;
 }
 
 static int engine_retile(Engine &e, int tl) {
@@ -118,7 +120,7 @@ int engine_alloc(Engine &e, Ctx *ctx, int precision, int stencil, int32_t grid_w
     e.h_desc = descs;
     e.n_inst = (int)descs.size();
     e.data_elems = data_elems;
-    e.chunk = precision == PASC_FP32_BULK ? 8 : 4;
+    e.chunk = precision == PASC_FP32_BULK ? 4 : 2;
     e.block = 256;
     int64_t se = 0;
     for (auto &d : descs) se = std::max(se, d.goff + (int64_t)d.K * d.ld);

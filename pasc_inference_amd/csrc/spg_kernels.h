@@ -30,26 +30,23 @@ struct PassArgs {
     GroupView grp;
 };
 
-// 2*(gamma_t - gamma_{t-1})[t>0] + 2*(gamma_t - gamma_{t+1})[t<T-1]: row of A0
-// (blockgraphsparse.cpp:60-89) in difference form, which keeps FP32 rounding relative to the
-// differences rather than to 4*gamma_t.
-template <typename C, int W>
-__device__ __forceinline__ C lap_time(const C (&v)[W], int j, int64_t t, int64_t T) {
-    C a = C(0);
-    if (t > 0) a += v[j] - v[j - 1];
-    if (t < T - 1) a += v[j] - v[j + 1];
-    return C(2) * a;
-}
+template <int CH>
+constexpr int halo_threads() { return (3 + CH - 1) / CH; }
 
-template <typename S, typename C, int W>
-__device__ __forceinline__ void load_window(const S *__restrict__ row, int64_t tw0, int64_t T,
-                                            int64_t ld, C (&v)[W]) {
+template <int NT, int CH>
+constexpr int tile_steps() { return (NT - 2 * halo_threads<CH>()) * CH; }
+
+// Loads CH consecutive steps [t0, t0+CH) of one row (t0 a multiple of the vector width);
+// steps outside [0, T) read as 0.
+template <typename S, typename C, int CH>
+__device__ __forceinline__ void load_chunk(const S *__restrict__ row, int64_t t0, int64_t T,
+                                           int64_t ld, C (&v)[CH]) {
     using VT = V16<S>;
     constexpr int VW = VT::n;
-    static_assert(W % VW == 0, "window must be whole vectors");
+    static_assert(CH % VW == 0, "chunk must be whole vectors");
 #pragma unroll
-    for (int q = 0; q < W / VW; q++) {
-        const int64_t t = tw0 + q * VW;
+    for (int q = 0; q < CH / VW; q++) {
+        const int64_t t = t0 + q * VW;
         if (t >= 0 && t < ld) {
             const typename VT::type vec = *reinterpret_cast<const typename VT::type *>(row + t);
 #pragma unroll
@@ -61,15 +58,49 @@ __device__ __forceinline__ void load_window(const S *__restrict__ row, int64_t t
     }
 }
 
+// Neighbour values across chunk edges: every thread publishes its first and last step of each
+// row; thread i then reads step t0-1 from thread i-1 and step t0+CH from thread i+1.
+template <typename C, int K, int CH, int NT>
+__device__ __forceinline__ void exchange_edges(const C (&v)[K][CH], C *lds, C (&left)[K],
+                                               C (&right)[K]) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        lds[(2 * k) * NT + tid] = v[k][0];
+        lds[(2 * k + 1) * NT + tid] = v[k][CH - 1];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        left[k] = tid > 0 ? lds[(2 * k + 1) * NT + tid - 1] : C(0);
+        right[k] = tid < NT - 1 ? lds[(2 * k) * NT + tid + 1] : C(0);
+    }
+    __syncthreads();
+}
+
+template <typename C>
+__device__ __forceinline__ C lap2(C v, C vm, C vp, int64_t t, int64_t T) {
+    // row of A0 (blockgraphsparse.cpp:60-89) in difference form:
+    // 2*(v_t - v_{t-1})[t>0] + 2*(v_t - v_{t+1})[t<T-1]
+    C a = C(0);
+    if (t > 0) a += v - vm;
+    if (t < T - 1) a += v - vp;
+    return C(2) * a;
+}
+
 template <typename S, typename C, int K, int CH, int NT>
 __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int chunk,
-                                             double *lds) {
-    constexpr int W = CH + 8;  // window starts 4 steps before the first owned step
+                                             double *lds_red, C *lds_x) {
+    constexpr int HT = halo_threads<CH>();
+    constexpr int TILE = tile_steps<NT, CH>();
     const InstDesc &D = A.desc[inst];
     const InstState &st = A.state[inst];
     const int64_t T = D.T;
     const int64_t ld = D.ld;
-    const int64_t t0 = (int64_t)chunk * (NT * CH) + (int64_t)threadIdx.x * CH;
+    const int tid = threadIdx.x;
+    const int64_t T0 = (int64_t)chunk * TILE;
+    const int64_t t0 = T0 - (int64_t)HT * CH + (int64_t)tid * CH;
+    const bool owner = tid >= HT && tid < NT - HT;  // owned steps are stored and reduced
     const int par = st.parity;
     const S *src = reinterpret_cast<const S *>(par ? A.gamma1 : A.gamma0) + D.goff;
     S *dst = reinterpret_cast<S *>(par ? A.gamma0 : A.gamma1) + D.goff;
@@ -83,124 +114,147 @@ __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int ch
     const C a_dir = C(st.alpha_dir);
     const bool first = st.first != 0;
 
-    double acc[NPART] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    // data term q = bscale*(x - theta_k)^2 = -b (zero when held out or outside [0,T))
+    C X[CH];
+    load_chunk<S, C, CH>(xr, t0, T, ld, X);
+    bool obs[CH];
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+        const int64_t t = t0 + c;
+        bool o = (t >= 0 && t < T);
+        if (o && D.heldout) o = D.heldout[t] == 0;
+        obs[c] = o;
+    }
+    C th[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) th[k] = C(D.theta[k]);
+    auto qterm = [&](int k, int c) -> C {
+        const C dx = X[c] - th[k];
+        return obs[c] ? bscale * dx * dx : C(0);
+    };
 
-    if (t0 < T) {
-        C X[W];
-        load_window<S, C, W>(xr, t0 - 4, T, ld, X);
-        C th[K];
+    // level 1: gamma_i = gamma_{i-1} + beta_{i-1} (P(gamma_{i-1} - alpha_{i-1} g_{i-1}) - gamma_{i-1})
+    C G1[K][CH];
+    {
+        C G[K][CH];
 #pragma unroll
-        for (int k = 0; k < K; k++) th[k] = C(D.theta[k]);
-        // data term q = bscale*(x - theta_k)^2 = -b (zero when held out or outside [0,T))
-        bool obs[W];
+        for (int k = 0; k < K; k++) load_chunk<S, C, CH>(src + k * ld, t0, T, ld, G[k]);
+        C gl[K], gr[K];
+        exchange_edges<C, K, CH, NT>(G, lds_x, gl, gr);
 #pragma unroll
-        for (int j = 0; j < W; j++) {
-            const int64_t t = t0 - 4 + j;
-            bool o = (t >= 0 && t < T);
-            if (o && D.heldout) o = D.heldout[t] == 0;
-            obs[j] = o;
-        }
-        C G[K][W];
-#pragma unroll
-        for (int k = 0; k < K; k++) load_window<S, C, W>(src + k * ld, t0 - 4, T, ld, G[k]);
-
-        // level 1: gamma_i on window positions [2, W-2)
-        C G1[K][W];
-#pragma unroll
-        for (int j = 2; j < W - 2; j++) {
-            const int64_t t = t0 - 4 + j;
+        for (int c = 0; c < CH; c++) {
+            const int64_t t = t0 + c;
             C y[K];
             if (first) {
 #pragma unroll
-                for (int k = 0; k < K; k++) y[k] = G[k][j];
+                for (int k = 0; k < K; k++) y[k] = G[k][c];
                 project_simplex<K, C>(y);
 #pragma unroll
-                for (int k = 0; k < K; k++) G1[k][j] = y[k];
+                for (int k = 0; k < K; k++) G1[k][c] = y[k];
             } else {
 #pragma unroll
                 for (int k = 0; k < K; k++) {
-                    const C dx = X[j] - th[k];
-                    const C q = obs[j] ? bscale * dx * dx : C(0);
-                    const C g = alpha * lap_time<C, W>(G[k], j, t, T) + q;
-                    y[k] = G[k][j] - a_apply * g;
+                    const C vm = c > 0 ? G[k][c - 1] : gl[k];
+                    const C vp = c < CH - 1 ? G[k][c + 1] : gr[k];
+                    const C g = alpha * lap2(G[k][c], vm, vp, t, T) + qterm(k, c);
+                    y[k] = G[k][c] - a_apply * g;
                 }
                 project_simplex<K, C>(y);
 #pragma unroll
-                for (int k = 0; k < K; k++) G1[k][j] = G[k][j] + b_apply * (y[k] - G[k][j]);
+                for (int k = 0; k < K; k++) G1[k][c] = G[k][c] + b_apply * (y[k] - G[k][c]);
             }
         }
-        // level 2: g_i and d_i on [3, W-3)
-        C Dd[K][W];
-        C Gi[K][CH];
+    }
+    // level 2: g_i and d_i = P(gamma_i - alpha_i g_i) - gamma_i
+    C Gi[K][CH], Dd[K][CH], off[CH];
+    {
+        C gl[K], gr[K];
+        exchange_edges<C, K, CH, NT>(G1, lds_x, gl, gr);
 #pragma unroll
-        for (int j = 3; j < W - 3; j++) {
-            const int64_t t = t0 - 4 + j;
-            C y[K];
+        for (int c = 0; c < CH; c++) {
+            const int64_t t = t0 + c;
+            C y[K], z[K];
 #pragma unroll
             for (int k = 0; k < K; k++) {
-                const C dx = X[j] - th[k];
-                const C q = obs[j] ? bscale * dx * dx : C(0);
-                const C g = alpha * lap_time<C, W>(G1[k], j, t, T) + q;
-                if (j >= 4 && j < W - 4) Gi[k][j - 4] = g;
-                y[k] = G1[k][j] - a_dir * g;
+                const C vm = c > 0 ? G1[k][c - 1] : gl[k];
+                const C vp = c < CH - 1 ? G1[k][c + 1] : gr[k];
+                Gi[k][c] = alpha * lap2(G1[k][c], vm, vp, t, T) + qterm(k, c);
+                y[k] = G1[k][c] - a_dir * Gi[k][c];
+                z[k] = y[k];
             }
-            project_simplex<K, C>(y);
+            const C tau = project_simplex<K, C>(z);
+            // <g, d> = -(|d|^2 + sum_{k off the support} (tau - y_k) gamma_k) / alpha holds for the
+            // projected-gradient step (it follows from g = (gamma - y)/alpha and sum_k d_k = 0);
+            // every term is non-negative, so the computed value cannot lose its sign to rounding
+            // the way the plain dot product does once d is tiny.
+            C o = C(0);
 #pragma unroll
-            for (int k = 0; k < K; k++) Dd[k][j] = y[k] - G1[k][j];
+            for (int k = 0; k < K; k++) {
+                Dd[k][c] = z[k] - G1[k][c];
+                if (y[k] <= tau) o += (tau - y[k]) * G1[k][c];
+            }
+            off[c] = o;
         }
-        // level 3: owned steps [4, W-4)
+    }
+    // level 3: reductions on the owned steps, store gamma_i
+    double acc[NPART] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    {
+        C dl[K], dr[K];
+        exchange_edges<C, K, CH, NT>(Dd, lds_x, dl, dr);
+        if (owner) {
+            const double inv_a = 1.0 / (double)a_dir;
 #pragma unroll
-        for (int j = 4; j < W - 4; j++) {
-            const int64_t t = t0 - 4 + j;
-            if (t < T) {
-                C z[K];
-                // <g, d> with the per-step common mode g_0t removed: every block of d sums to 0,
-                // so the value is unchanged, but the rounding of sum_k d_kt no longer multiplies
-                // the large data term (which otherwise stalls SPG near 1e-9 KKT).
-                const C gc = Gi[0][j - 4];
+            for (int c = 0; c < CH; c++) {
+                const int64_t t = t0 + c;
+                if (t < T) {
+                    C z[K];
+                    double ddt = 0.0;
 #pragma unroll
-                for (int k = 0; k < K; k++) {
-                    const C d = Dd[k][j];
-                    const C Ad = alpha * lap_time<C, W>(Dd[k], j, t, T);
-                    const C g = Gi[k][j - 4];
-                    const C dx = X[j] - th[k];
-                    const C q = obs[j] ? bscale * dx * dx : C(0);
-                    acc[0] += (double)d * (double)d;
-                    acc[1] += (double)d * (double)Ad;
-                    acc[2] += ((double)g - (double)gc) * (double)d;
-                    acc[3] += 0.5 * ((double)g + (double)q) * (double)G1[k][j];
-                    z[k] = G1[k][j] - s_kkt * g;
-                }
-                project_simplex<K, C>(z);
+                    for (int k = 0; k < K; k++) {
+                        const C d = Dd[k][c];
+                        const C g = Gi[k][c];
+                        ddt += (double)d * (double)d;
+                        if (t < T - 1) {
+                            // d'A d = alpha * sum over edges (t, t+1) of 2 (d_{t+1} - d_t)^2
+                            const C dn = c < CH - 1 ? Dd[k][c + 1] : dr[k];
+                            const double e = (double)dn - (double)d;
+                            acc[1] += 2.0 * (double)alpha * e * e;
+                        }
+                        acc[3] += 0.5 * ((double)g + (double)qterm(k, c)) * (double)G1[k][c];
+                        z[k] = G1[k][c] - s_kkt * g;
+                    }
+                    acc[0] += ddt;
+                    acc[2] -= inv_a * (ddt + (double)off[c]);
+                    project_simplex<K, C>(z);
 #pragma unroll
-                for (int k = 0; k < K; k++) {
-                    const double e = fabs((double)G1[k][j] - (double)z[k]);
-                    acc[4] = (e > acc[4] || e != e) ? e : acc[4];
+                    for (int k = 0; k < K; k++) {
+                        const double e = fabs((double)G1[k][c] - (double)z[k]);
+                        acc[4] = (e > acc[4] || e != e) ? e : acc[4];
+                    }
                 }
             }
-        }
-        // store gamma_i for the owned steps
-        using VT = V16<S>;
-        constexpr int VW = VT::n;
+            using VT = V16<S>;
+            constexpr int VW = VT::n;
 #pragma unroll
-        for (int k = 0; k < K; k++) {
+            for (int k = 0; k < K; k++) {
 #pragma unroll
-            for (int q = 0; q < CH / VW; q++) {
-                const int64_t t = t0 + q * VW;
-                if (t + VW <= T) {
-                    typename VT::type vec;
+                for (int q = 0; q < CH / VW; q++) {
+                    const int64_t t = t0 + q * VW;
+                    if (t + VW <= T) {
+                        typename VT::type vec;
 #pragma unroll
-                    for (int i = 0; i < VW; i++) VT::set(vec, i, S(G1[k][4 + q * VW + i]));
-                    *reinterpret_cast<typename VT::type *>(dst + k * ld + t) = vec;
-                } else {
+                        for (int i = 0; i < VW; i++) VT::set(vec, i, S(G1[k][q * VW + i]));
+                        *reinterpret_cast<typename VT::type *>(dst + k * ld + t) = vec;
+                    } else {
 #pragma unroll
-                    for (int i = 0; i < VW; i++)
-                        if (t + i < T) dst[k * ld + t + i] = S(G1[k][4 + q * VW + i]);
+                        for (int i = 0; i < VW; i++)
+                            if (t + i < T) dst[k * ld + t + i] = S(G1[k][q * VW + i]);
+                    }
                 }
             }
         }
     }
-    block_reduce_part<NT>(acc, lds);
+    block_reduce_part<NT>(acc, lds_red);
     if (threadIdx.x == 0) {
         double *o = A.part + (int64_t)(D.tile0 + chunk) * NPART;
 #pragma unroll
@@ -210,12 +264,13 @@ __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int ch
 
 template <typename S, typename C, int K, int CH, int NT>
 __global__ __launch_bounds__(NT) void spg_pass_1d(PassArgs A) {
-    __shared__ double lds[NPART * (NT / WAVE)];
+    __shared__ double lds_red[NPART * (NT / WAVE)];
+    __shared__ C lds_x[2 * K * NT];
     const int total = A.grp.off[A.grp.n];
     for (int j = blockIdx.x; j < total; j += gridDim.x) {
         int inst, chunk;
         find_tile(A.grp, j, inst, chunk);
-        pass_tile_1d<S, C, K, CH, NT>(A, inst, chunk, lds);
+        pass_tile_1d<S, C, K, CH, NT>(A, inst, chunk, lds_red, lds_x);
     }
 }
 
@@ -311,26 +366,31 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_dir(GridArgs A) {
             }
             C q[K];
             grid_dataterm<C, K>(D, A.x, r, is_f32, q);
-            C y[K], z[K], g[K];
+            C y[K], p[K], z[K], g[K];
             const C alpha = C(D.alpha), a_dir = C(st.alpha_dir), s = C(D.s_kkt);
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 g[k] = alpha * lap[k] + q[k];
                 y[k] = g0[k] - a_dir * g[k];
+                p[k] = y[k];
                 z[k] = g0[k] - s * g[k];
             }
-            project_simplex<K, C>(y);
+            const C tau = project_simplex<K, C>(p);
             project_simplex<K, C>(z);
+            double ddt = 0.0, off = 0.0;
 #pragma unroll
             for (int k = 0; k < K; k++) {
-                const C d = y[k] - g0[k];
+                const C d = p[k] - g0[k];
                 dst[k * D.ld + r] = S(g0[k]);
                 dr[k * D.ld + r] = S(d);
-                acc[2] += ((double)g[k] - (double)g[0]) * (double)d;  // common mode removed
+                ddt += (double)d * (double)d;
+                if (y[k] <= tau) off += (double)(tau - y[k]) * (double)g0[k];
                 acc[3] += 0.5 * ((double)g[k] + (double)q[k]) * (double)g0[k];
                 const double e = fabs((double)g0[k] - (double)z[k]);
                 acc[4] = (e > acc[4] || e != e) ? e : acc[4];
             }
+            // <g, d> in the sign-safe form of the projected-gradient step (see pass_tile_1d)
+            acc[2] -= (ddt + off) / (double)a_dir;
         }
         block_reduce_part<NT>(acc, lds);
         if (threadIdx.x == 0) {
@@ -360,17 +420,22 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_ad(GridArgs A) {
             const int64_t row = r / W, col = r - row * W;
             const int64_t nb[4] = {row > 0 ? r - W : -1, col > 0 ? r - 1 : -1,
                                    col < W - 1 ? r + 1 : -1, r + W < R ? r + W : -1};
-            const C alpha = C(D.alpha);
+            (void)nb;
+            const double alpha = D.alpha;
 #pragma unroll
             for (int k = 0; k < K; k++) {
-                const C d = C(dr[k * D.ld + r]);
-                C lap = C(0);
-#pragma unroll
-                for (int e = 0; e < 4; e++)
-                    if (nb[e] >= 0) lap += d - C(dr[k * D.ld + nb[e]]);
-                const C Ad = alpha * lap;
-                acc[0] += (double)d * (double)d;
-                acc[1] += (double)d * (double)Ad;
+                const double d = (double)dr[k * D.ld + r];
+                acc[0] += d * d;
+                // d'A d = alpha * sum over grid edges of (d_r - d_s)^2; each edge is counted
+                // once, from its left / upper end
+                if (col < W - 1) {
+                    const double e = d - (double)dr[k * D.ld + r + 1];
+                    acc[1] += alpha * e * e;
+                }
+                if (r + W < R) {
+                    const double e = d - (double)dr[k * D.ld + r + W];
+                    acc[1] += alpha * e * e;
+                }
             }
         }
         block_reduce_part<NT>(acc, lds);

@@ -61,7 +61,7 @@ __global__ __launch_bounds__(256) void spg_step(const InstDesc *desc, InstState 
     if (!stop && dd == 0.0) { stop = true; reason = PASC_REASON_STATIONARY; }
     if (!stop && P.fp32_phase) {
         // FP32 phase ends at the switch threshold or when the KKT residual stagnates
-        if (kkt < 0.5 * s.best_kkt) { s.best_kkt = kkt; s.stall = 0; }
+        if (kkt < 0.9 * s.best_kkt) { s.best_kkt = kkt; s.stall = 0; }
         else s.stall += 1;
         if ((P.fp32_switch > 0.0 && kkt <= P.fp32_switch) || s.stall >= P.stall_window) {
             stop = true;

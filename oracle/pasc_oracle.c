@@ -366,14 +366,18 @@ int or_spg_solve(double *x, const double *b, int64_t T, int64_t R, int K, double
         fx_max = fs_get_max(&fs); /* :176 */
 
         /* step-size from the generalised Armijo condition (:182-195) */
-        xi = (fx_max - fx) / dAd;
-        beta_bar = -gd / dAd;
-        beta_hat = p->gamma * beta_bar + sqrt(p->gamma * p->gamma * beta_bar * beta_bar + 2 * xi);
-        if (beta_hat < p->sigma1) beta_hat = p->sigma1;
-        if (beta_hat < p->sigma2)
-            beta = beta_hat;
-        else
+        if (p->safeguard && !(dAd > 0.0)) {
             beta = p->sigma2;
+        } else {
+            xi = (fx_max - fx) / dAd;
+            beta_bar = -gd / dAd;
+            beta_hat = p->gamma * beta_bar + sqrt(p->gamma * p->gamma * beta_bar * beta_bar + 2 * xi);
+            if (beta_hat < p->sigma1) beta_hat = p->sigma1;
+            if (beta_hat < p->sigma2)
+                beta = beta_hat;
+            else
+                beta = p->sigma2;
+        }
 
         /* x += beta*d; g += beta*Ad (:200-201) */
         v_axpy(x, beta, d, n, nthreads);
@@ -388,7 +392,7 @@ int or_spg_solve(double *x, const double *b, int64_t T, int64_t R, int K, double
         fs_update(&fs, fx);
 
         /* :216 */
-        alpha_bb = dd / dAd;
+        alpha_bb = (p->safeguard && !(dAd > 0.0)) ? p->alphainit : dd / dAd;
 
         /* stopping criteria :260-274 */
         if (p->stop_difff && fabs(fx - fx_old) < p->eps) { reason = 1; break; }

@@ -79,6 +79,8 @@ typedef struct {
     int fx_identity;      /* f_new = f + beta*(gd + beta*dAd/2), spgqpsolver.h:835-842 */
     int gd_centered;      /* gd = sum_t sum_k (g_kt - g_0t) d_kt (equal in exact arithmetic,
                              since sum_k d_kt = 0; removes the common-mode rounding) */
+    int safeguard;        /* dAd <= 0 (d in the null space of A): beta = sigma2 and the BB step
+                             restarts at alphainit, where the reference divides by zero */
 } or_spg_params;
 
 typedef struct {

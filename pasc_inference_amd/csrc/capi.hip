@@ -148,7 +148,7 @@ void pasc_spg_default_params(pasc_spg_params *p) {
     p->check_every = 1;
     p->use_graph = 0;
     p->fp32_switch = 0.0;
-    p->stall_window = 100;
+    p->stall_window = 400;
     p->polish_maxit = 0;
 }
 

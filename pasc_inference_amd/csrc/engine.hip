@@ -144,7 +144,9 @@ int engine_alloc(Engine &e, Ctx *ctx, int precision, int stencil, int32_t grid_w
     HIP_TRY(hipMalloc(&e.d_desc, sizeof(InstDesc) * std::max(e.n_inst, 1)));
     HIP_TRY(hipMalloc(&e.d_state, sizeof(InstState) * std::max(e.n_inst, 1)));
     HIP_TRY(hipMalloc(&e.d_nactive, sizeof(int32_t)));
-    HIP_TRY(hipHostMalloc((void **)&e.h_nactive, sizeof(int32_t), hipHostMallocDefault));
+    // pinned poll area: [0] active instances, [1 + g] active tiles of K group g
+    HIP_TRY(hipHostMalloc((void **)&e.h_nactive, sizeof(int32_t) * (1 + PASC_MAX_K),
+                          hipHostMallocDefault));
     e.h_parity.assign(e.n_inst, 0);
     // K groups
     for (int K = 1; K <= PASC_MAX_K; K++) {
@@ -375,6 +377,8 @@ static int run_phase(Engine &e, const PhaseCfg &ph, const pasc_spg_params &p, in
         grids.push_back((int)std::max<int64_t>(1, std::min<int64_t>(g.max_tiles, cap)));
     }
     const int every = std::max(1, p.check_every);
+    std::vector<int32_t *> d_totals;
+    for (auto &g : e.groups) d_totals.push_back(g.d_off + g.n);
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     std::vector<hipEvent_t> evs;
@@ -401,8 +405,27 @@ static int run_phase(Engine &e, const PhaseCfg &ph, const pasc_spg_params &p, in
         }
         passes += n;
         HIP_TRY(hipMemcpyAsync(e.h_nactive, e.d_nactive, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        for (size_t gi = 0; gi < d_totals.size(); gi++)
+            HIP_TRY(hipMemcpyAsync(e.h_nactive + 1 + gi, d_totals[gi], sizeof(int32_t),
+                                   hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         if (*e.h_nactive == 0) break;
+        // shrink the persistent grids once most tiles have converged (idle workgroups still cost
+        // a dispatch each); the graph is re-captured with the new launch shapes
+        bool shrink = false;
+        for (size_t gi = 0; gi < grids.size(); gi++) {
+            const int need = std::max(1, (int)e.h_nactive[1 + gi]);
+            if (grids[gi] > 2 * need && grids[gi] > 64) {
+                grids[gi] = std::max(need, 64);
+                shrink = true;
+            }
+        }
+        if (shrink && exec) {
+            (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
+            exec = nullptr;
+            graph = nullptr;
+        }
     }
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -454,7 +477,9 @@ static StepParams make_step(const pasc_spg_params &p, bool fp32_phase) {
     sp.kkt_tol = p.kkt_tol;
     sp.fp32_phase = fp32_phase ? 1 : 0;
     sp.stall_window = std::max(1, p.stall_window);
-    sp.fp32_switch = p.fp32_switch;
+    // hand over to FP64 once the FP32 residual is within 30x of the target (the FP32 floor of
+    // the residual is ~1e-5 at epsilon 100), or on stagnation
+    sp.fp32_switch = p.fp32_switch > 0.0 ? p.fp32_switch : 30.0 * p.kkt_tol;
     return sp;
 }
 

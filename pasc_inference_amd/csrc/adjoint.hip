@@ -1,10 +1,282 @@
-// Adjoint of the converged gamma (placeholder until the CG kernels land).
+// Adjoint of the converged gamma step (1-D time graph, xdim = 1).
+//
+// At the minimiser gamma* of  min 1/2 g'Ag - b'g  over the product of simplices, let F be the
+// free set {(k,t): gamma_kt > 0} and P the orthogonal projector onto
+// {v : v = 0 off F, sum_{k in F_t} v_kt = 0 for every t}.  The KKT conditions P(A gamma - b) = 0
+// differentiate to  (P A P) dgamma = -P(dA gamma - db).  For the held-out loss
+//   J = sum_{t in M} (x_t - sum_k theta_k gamma_kt)^2
+// one CG solve of (P A P) lambda = P dJ/dgamma gives every parameter derivative:
+//   dJ/dlog(eps) = -lambda' A gamma                      (A = eps/T A0, so dA/dlog eps = A)
+//   dJ/dtheta_k  = dJ/dtheta_k|direct + sum_{t with data} lambda_kt * 2 (x_t - theta_k) / T
+// All vectors are FP64, k-major with row stride ld, one thread per time step.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
 #include "device_common.h"
 
 namespace pasc {
 
-int engine_adjoint(Engine &, int, const uint8_t *, double, int, double *, double *, double *, int *) {
-    return fail(PASC_ERR_STATE, "adjoint: not available in this build");
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(PASC_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));     \
+    } while (0)
+
+constexpr int ANT = 256;
+constexpr double FREE_EPS = 1e-12;  // gamma entries at or below this are on the bound
+
+struct AdjArgs {
+    const double *gam;      // gamma* (k-major, stride ld)
+    const double *x;        // data (stride-1 row)
+    const uint8_t *mask;    // held-out steps (1 = in the loss)
+    const uint8_t *qmask;   // steps without a data term in the QP (problem's held-out mask)
+    uint8_t *freeset;       // [K][ld] 1 = free
+    int64_t T, ld;
+    int K;
+    double alpha, bscale;
+    double theta[PASC_MAX_K];
+};
+
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV], double *lds, double *out) {
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+#pragma unroll
+    for (int i = 0; i < NV; i++) v[i] = wave_sum(v[i]);
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; i++) lds[w * NV + i] = v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < NV; i++) {
+            double s = 0.0;
+            for (int q = 0; q < ANT / WAVE; q++) s += lds[q * NV + i];
+            out[(int64_t)blockIdx.x * NV + i] = s;
+        }
+    }
+}
+
+// P applied to the K values of one time step (in place); free flags per k.
+__device__ inline void project_face(double *v, const bool *fr, int K) {
+    int nf = 0;
+    double s = 0.0;
+    for (int k = 0; k < K; k++)
+        if (fr[k]) { nf++; s += v[k]; }
+    const double m = nf > 0 ? s / nf : 0.0;
+    for (int k = 0; k < K; k++) v[k] = (fr[k] && nf > 1) ? v[k] - m : 0.0;
+}
+
+// r = P dJ/dgamma; partials: [0] J, [1] r'r, [2 + k] direct dJ/dtheta_k
+__global__ __launch_bounds__(ANT) void adj_init(AdjArgs a, double *r, double *part) {
+    __shared__ double lds[(2 + PASC_MAX_K) * (ANT / WAVE)];
+    const int64_t t = (int64_t)blockIdx.x * ANT + threadIdx.x;
+    double v[2 + PASC_MAX_K];
+    for (int i = 0; i < 2 + PASC_MAX_K; i++) v[i] = 0.0;
+    if (t < a.T) {
+        double g[PASC_MAX_K], dj[PASC_MAX_K];
+        bool fr[PASC_MAX_K];
+        double rec = 0.0;
+        for (int k = 0; k < a.K; k++) {
+            g[k] = a.gam[k * a.ld + t];
+            fr[k] = g[k] > FREE_EPS;
+            a.freeset[k * a.ld + t] = fr[k] ? 1 : 0;
+            rec += a.theta[k] * g[k];
+        }
+        const bool held = a.mask[t] != 0;
+        const double res = held ? a.x[t] - rec : 0.0;
+        for (int k = 0; k < a.K; k++) {
+            dj[k] = -2.0 * res * a.theta[k];
+            v[2 + k] = -2.0 * res * g[k];
+        }
+        v[0] = res * res;
+        project_face(dj, fr, a.K);
+        for (int k = 0; k < a.K; k++) {
+            r[k * a.ld + t] = dj[k];
+            v[1] += dj[k] * dj[k];
+        }
+    }
+    block_sum<2 + PASC_MAX_K>(v, lds, part);
+}
+
+// q = P A P p (p already in range(P)); partial p'q
+__global__ __launch_bounds__(ANT) void adj_matvec(AdjArgs a, const double *p, double *q,
+                                                  double *part) {
+    __shared__ double lds[ANT / WAVE];
+    const int64_t t = (int64_t)blockIdx.x * ANT + threadIdx.x;
+    double v[1] = {0.0};
+    if (t < a.T) {
+        double w[PASC_MAX_K];
+        bool fr[PASC_MAX_K];
+        for (int k = 0; k < a.K; k++) {
+            const double *row = p + k * a.ld;
+            double l = 0.0;
+            if (t > 0) l += row[t] - row[t - 1];
+            if (t < a.T - 1) l += row[t] - row[t + 1];
+            w[k] = a.alpha * 2.0 * l;
+            fr[k] = a.freeset[k * a.ld + t] != 0;
+        }
+        project_face(w, fr, a.K);
+        for (int k = 0; k < a.K; k++) {
+            q[k * a.ld + t] = w[k];
+            v[0] += p[k * a.ld + t] * w[k];
+        }
+    }
+    block_sum<1>(v, lds, part);
+}
+
+// lam += s p; r -= s q; partial r'r
+__global__ __launch_bounds__(ANT) void adj_axpy(AdjArgs a, double s, const double *p,
+                                                const double *q, double *lam, double *r,
+                                                double *part) {
+    __shared__ double lds[ANT / WAVE];
+    const int64_t t = (int64_t)blockIdx.x * ANT + threadIdx.x;
+    double v[1] = {0.0};
+    if (t < a.T) {
+        for (int k = 0; k < a.K; k++) {
+            const int64_t i = k * a.ld + t;
+            lam[i] += s * p[i];
+            r[i] -= s * q[i];
+            v[0] += r[i] * r[i];
+        }
+    }
+    block_sum<1>(v, lds, part);
+}
+
+__global__ __launch_bounds__(ANT) void adj_pupdate(AdjArgs a, double b, const double *r,
+                                                   double *p) {
+    const int64_t t = (int64_t)blockIdx.x * ANT + threadIdx.x;
+    if (t >= a.T) return;
+    for (int k = 0; k < a.K; k++) {
+        const int64_t i = k * a.ld + t;
+        p[i] = r[i] + b * p[i];
+    }
+}
+
+// partials: [0] lambda' A gamma, [1 + k] sum_{t not held} lambda_kt * 2 (x_t - theta_k) * bscale
+__global__ __launch_bounds__(ANT) void adj_final(AdjArgs a, const double *lam, double *part) {
+    __shared__ double lds[(1 + PASC_MAX_K) * (ANT / WAVE)];
+    const int64_t t = (int64_t)blockIdx.x * ANT + threadIdx.x;
+    double v[1 + PASC_MAX_K];
+    for (int i = 0; i < 1 + PASC_MAX_K; i++) v[i] = 0.0;
+    if (t < a.T) {
+        const bool nodata = a.qmask && a.qmask[t] != 0;  // b_kt does not depend on theta there
+        for (int k = 0; k < a.K; k++) {
+            const double *row = a.gam + k * a.ld;
+            double l = 0.0;
+            if (t > 0) l += row[t] - row[t - 1];
+            if (t < a.T - 1) l += row[t] - row[t + 1];
+            const double lk = lam[k * a.ld + t];
+            v[0] += lk * a.alpha * 2.0 * l;
+            if (!nodata) v[1 + k] = lk * 2.0 * (a.x[t] - a.theta[k]) * a.bscale;
+        }
+    }
+    block_sum<1 + PASC_MAX_K>(v, lds, part);
+}
+
+static int sum_partials(hipStream_t s, const double *d_part, int blocks, int nv,
+                        std::vector<double> &out) {
+    std::vector<double> h((size_t)blocks * nv);
+    HIP_TRY(hipMemcpyAsync(h.data(), d_part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    out.assign(nv, 0.0);
+    for (int b = 0; b < blocks; b++)
+        for (int i = 0; i < nv; i++) out[i] += h[(size_t)b * nv + i];
+    return PASC_OK;
+}
+
+template <typename A, typename B>
+__global__ __launch_bounds__(256) void adj_convert(const A *src, B *dst, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = B(src[i]);
+}
+
+int engine_adjoint(Engine &e, int inst, const uint8_t *mask_host, double cg_tol, int cg_maxit,
+                   double *loss, double *dlogeps, double *dtheta, int *cg_it) {
+    if (e.stencil != STENCIL_TIME) return fail(PASC_ERR_ARG, "adjoint: 1-D time graph only");
+    const InstDesc &D = e.h_desc[inst];
+    if (D.xdim != 1) return fail(PASC_ERR_ARG, "adjoint: xdim must be 1");
+    hipStream_t s = (hipStream_t)e.ctx->stream;
+    const int64_t T = D.T, ld = D.ld, n = (int64_t)D.K * ld;
+    const int blocks = (int)((T + ANT - 1) / ANT);
+    const int par = e.h_parity[inst];
+    double *buf = nullptr;
+    uint8_t *fr = nullptr, *mk = nullptr;
+    double *part = nullptr;
+    // gamma (FP64 copy when the state is FP32), lambda, r, p, q
+    HIP_TRY(hipMalloc(&buf, sizeof(double) * 5 * (size_t)n));
+    HIP_TRY(hipMalloc(&fr, (size_t)n));
+    HIP_TRY(hipMalloc(&mk, (size_t)T));
+    HIP_TRY(hipMalloc(&part, sizeof(double) * (size_t)blocks * (2 + PASC_MAX_K)));
+    double *gam = buf, *lam = buf + n, *r = buf + 2 * n, *p = buf + 3 * n, *q = buf + 4 * n;
+    int rc = PASC_OK;
+    auto cleanup = [&]() {
+        (void)hipFree(buf);
+        (void)hipFree(fr);
+        (void)hipFree(mk);
+        (void)hipFree(part);
+    };
+    const unsigned cb = (unsigned)((n + 255) / 256);
+    if (e.cur64) {
+        hipLaunchKernelGGL((adj_convert<double, double>), dim3(cb), dim3(256), 0, s,
+                           (const double *)e.d_gamma64[par] + D.goff, gam, n);
+    } else if (e.precision == PASC_FP32_BULK) {
+        hipLaunchKernelGGL((adj_convert<float, double>), dim3(cb), dim3(256), 0, s,
+                           (const float *)e.d_gamma[par] + D.goff, gam, n);
+    } else {
+        hipLaunchKernelGGL((adj_convert<double, double>), dim3(cb), dim3(256), 0, s,
+                           (const double *)e.d_gamma[par] + D.goff, gam, n);
+    }
+    HIP_TRY(hipMemcpyAsync(mk, mask_host, (size_t)T, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemsetAsync(buf + n, 0, sizeof(double) * 4 * (size_t)n, s));
+    AdjArgs a;
+    a.gam = gam;
+    a.x = (const double *)e.d_x64 + D.xoff;
+    a.mask = mk;
+    a.qmask = D.heldout;
+    a.freeset = fr;
+    a.T = T;
+    a.ld = ld;
+    a.K = D.K;
+    a.alpha = D.alpha;
+    a.bscale = D.bscale;
+    for (int k = 0; k < PASC_MAX_K; k++) a.theta[k] = k < D.K ? D.theta[k] : 0.0;
+
+    std::vector<double> v;
+    hipLaunchKernelGGL(adj_init, dim3(blocks), dim3(ANT), 0, s, a, r, part);
+    HIP_TRY(hipGetLastError());
+    if ((rc = sum_partials(s, part, blocks, 2 + PASC_MAX_K, v))) { cleanup(); return rc; }
+    *loss = v[0];
+    double rr = v[1];
+    const double rr0 = rr;
+    std::vector<double> direct(v.begin() + 2, v.begin() + 2 + D.K);
+    HIP_TRY(hipMemcpyAsync(p, r, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
+    int it = 0;
+    // conjugate gradients on P A P (symmetric positive semi-definite on range(P))
+    while (it < cg_maxit && rr > cg_tol * cg_tol * rr0 && rr > 0.0) {
+        hipLaunchKernelGGL(adj_matvec, dim3(blocks), dim3(ANT), 0, s, a, p, q, part);
+        if ((rc = sum_partials(s, part, blocks, 1, v))) { cleanup(); return rc; }
+        const double pq = v[0];
+        if (!(pq > 0.0)) break;  // p in the null space of P A P: the system is consistent there
+        const double step = rr / pq;
+        hipLaunchKernelGGL(adj_axpy, dim3(blocks), dim3(ANT), 0, s, a, step, p, q, lam, r, part);
+        if ((rc = sum_partials(s, part, blocks, 1, v))) { cleanup(); return rc; }
+        const double rr_new = v[0];
+        hipLaunchKernelGGL(adj_pupdate, dim3(blocks), dim3(ANT), 0, s, a, rr_new / rr, r, p);
+        rr = rr_new;
+        it++;
+    }
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(adj_final, dim3(blocks), dim3(ANT), 0, s, a, lam, part);
+    if ((rc = sum_partials(s, part, blocks, 1 + PASC_MAX_K, v))) { cleanup(); return rc; }
+    *dlogeps = -v[0];
+    for (int k = 0; k < D.K; k++) dtheta[k] = direct[k] + v[1 + k];
+    if (cg_it) *cg_it = it;
+    cleanup();
+    if (!std::isfinite(*dlogeps)) return fail(PASC_ERR_NUMERIC, "adjoint: non-finite gradient");
+    return PASC_OK;
 }
 
 }  // namespace pasc

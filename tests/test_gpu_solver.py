@@ -97,7 +97,8 @@ def test_fp32_bulk_reaches_fp64_certified_kkt(ctx, oracle, eps):
     P.set_theta(theta)
     P.set_gamma_uniform()
     # the metric's target (KKT 1e-6), then the parity target (FP64-certified KKT 1e-9)
-    for tol, gtol in [(1e-6, 1e-4), (1e-9, 1e-6)]:
+    # KKT 1e-6 bounds the objective error only to a few 1e-8 (relative) at small epsilon
+    for tol, gtol, ftol in [(1e-6, 1e-4, 5e-8), (1e-9, 1e-6, 1e-8)]:
         P.set_gamma_uniform()
         r = P.gamma_solve(maxit=100000, stop_mask=capi.STOP_KKT, kkt_tol=tol)
         g = P.get_gamma()
@@ -106,7 +107,7 @@ def test_fp32_bulk_reaches_fp64_certified_kkt(ctx, oracle, eps):
         print(f"fp32-bulk eps={eps} tol={tol}: it32={r.it_fp32} it64={r.it_fp64} kkt={kkt:.2e} "
               f"gamma err={err:.2e} f rel={abs(f - rr.fx) / abs(rr.fx):.2e}")
         assert r.status == 0 and kkt <= tol
-        assert abs(f - rr.fx) <= 1e-8 * abs(rr.fx)
+        assert abs(f - rr.fx) <= ftol * abs(rr.fx)
         assert label_agreement(g, ref, K) >= 0.9999
         assert err <= gtol
 

@@ -429,3 +429,30 @@ def test_petsc_vec_roundtrip(tmp_path):
     assert raw[:4] == (1211214).to_bytes(4, "big")
     g = np.arange(12.0)
     np.testing.assert_array_equal(petsc_io.tmajor_to_kmajor(petsc_io.kmajor_to_tmajor(g, 3), 3), g)
+
+
+def test_unrolled_spg_reference_matches_central_differences(oracle):
+    """Pins the torch restatement of SPG used as the unrolled-gradient reference of the adjoint
+    test: its objective matches the oracle and its tangent matches central differences."""
+    import torch
+    from torch.func import jvp
+    from test_gpu_adjoint import _case, _torch_spg_loss
+    T = 600
+    x, mask, theta, eps = _case(T, seed=11)
+    K = theta.size
+
+    def J_oracle(le, th):
+        b = oracle.assemble_b(x, th, T, 1, K, heldout=mask)
+        g, _ = oracle.spg_solve(np.full(T * K, 1.0 / K), b, T, 1, K, np.exp(le) / T,
+                                maxit=100000, stop_difff=0, stop_kkt=1, kkt_tol=1e-13,
+                                kkt_scale=T, gd_centered=1)
+        return float(np.sum(((x - g.reshape(T, K) @ th)[mask == 1]) ** 2))
+
+    le = np.log(eps)
+    f = lambda a, b: _torch_spg_loss(x, mask, b, a, T, K, 600)  # noqa: E731
+    J, d = jvp(f, (torch.tensor(le, dtype=torch.float64), torch.tensor(theta)),
+               (torch.tensor(1.0, dtype=torch.float64), torch.zeros(K, dtype=torch.float64)))
+    assert abs(J.item() - J_oracle(le, theta)) <= 1e-9 * J.item()
+    h = 1e-4
+    fd = (J_oracle(le + h, theta) - J_oracle(le - h, theta)) / (2 * h)
+    assert abs(d.item() - fd) <= 1e-5 * max(abs(fd), 1e-3 * J.item())

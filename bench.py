@@ -37,6 +37,11 @@ def parse():
                     help="instances of the scan timed on the plain FP64 per-solve path")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=4)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch iterations one by one instead of replaying hipGraphs")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="only a fixed number of fused passes of the T=1e7 solve in each "
+                         "precision plus a copy kernel (for rocprofv3 traces and counters)")
     ap.add_argument("--verbose", action="store_true")
     return ap.parse_args()
 
@@ -62,7 +67,7 @@ def run_c2(args, ctx):
     P32 = Problem(ctx, x, K, precision=capi.FP32_BULK, epssqr=eps)
     P32.set_theta(theta)
     params = dict(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt, check_every=32,
-                  use_graph=1)
+                  use_graph=0 if args.no_graph else 1, kkt_every=4)
     for _ in range(args.warmup):
         P32.set_gamma_uniform()
         P32.gamma_solve(**params)
@@ -149,7 +154,7 @@ def run_scan(args, ctx):
     eps = SCAN_EPS
     n_inst = args.scan_series * eps.size
     params = dict(maxit=400000, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt, check_every=64,
-                  use_graph=1)
+                  use_graph=0 if args.no_graph else 1, kkt_every=4)
     B = Batch(ctx, xs, K, ths, eps, precision=capi.FP32_BULK)
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -216,14 +221,40 @@ def run_cpu_baseline(args, c2):
     )
 
 
+def profile_run(args):
+    """Fixed work for profilers: 64 fused FP32 passes, 64 FP64 passes, a 1 GiB copy kernel."""
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Context, Problem
+    x, _, theta, eps = c2_workload(args)
+    out = {}
+    with Context(0) as ctx:
+        out["copy_gbps"] = ctx.copy_bandwidth(1 << 30, 5)
+        for name, prec in (("fp32", capi.FP32_BULK), ("fp64", capi.FP64)):
+            P = Problem(ctx, x, theta.size, precision=prec, epssqr=eps)
+            P.set_theta(theta)
+            P.set_gamma_uniform()
+            P.set_profiling(True)
+            r = P.gamma_solve(maxit=64, stop_mask=0, polish_maxit=1)
+            P.close()
+            t = r.pass_time_fp32_s if prec == capi.FP32_BULK else r.pass_time_fp64_s
+            n = r.passes_fp32 if prec == capi.FP32_BULK else r.passes_fp64
+            out[name + "_pass_us"] = t / max(n, 1) * 1e6
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
+    if args.profile_run:
+        profile_run(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path "
               f"(requested --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
     from pasc_inference_amd.solver import Context
     with Context(0) as ctx:
+        copy_gbps = ctx.copy_bandwidth(1 << 30, 5)
+        log(args, f"device copy kernel: {copy_gbps:.0f} GB/s")
         c2 = run_c2(args, ctx)
         scan = None if args.no_scan else run_scan(args, ctx)
     cpu = None if args.no_cpu else run_cpu_baseline(args, c2)
@@ -264,6 +295,7 @@ def main():
             "own_min_bytes_per_launch": c2["own_bytes"],
             "own_min_frac": c2["own_bytes"] / c2["avg_pass"] / 1e9 / HBM_PEAK_GBS,
             "fp64_pass_avg_us": (c2["ref_pass64"] * 1e6) if c2["ref_pass64"] else None,
+            "device_copy_gbps": copy_gbps,
         },
         "scan": scan,
         "cpu_baseline": cpu,

@@ -97,10 +97,13 @@ typedef struct {
     int32_t check_every;  /* iterations between host convergence polls (default 1) */
     int32_t use_graph;    /* capture check_every iterations in a hipGraph (default 0) */
     double fp32_switch;   /* FP32-bulk: hand over to FP64 once the FP32 KKT <= this
-                             (default 0 = 30 * kkt_tol) */
+                             (default 0 = 100 * kkt_tol; never below the FP32
+                             floor 2.4e-7 * epsilon of the residual) */
     int32_t stall_window; /* FP32-bulk: iterations without a 10% improvement of the best KKT
-                             residual that end the FP32 phase (default 400) */
+                             residual that end the FP32 phase (default 200) */
     int32_t polish_maxit; /* FP32-bulk: maximum FP64 polish iterations (default maxit) */
+    int32_t kkt_every;    /* evaluate the KKT residual every kkt_every iterations (default 1);
+                             the KKT stop then fires at the first evaluated iteration */
 } pasc_spg_params;
 
 typedef struct {
@@ -241,6 +244,10 @@ int pasc_adjoint(pasc_problem *p, const uint8_t *mask, double cg_tol, int32_t cg
 /* ---- profiling: record hipEvents around every fused-pass launch of later solves (graph
  * capture is disabled meanwhile); the totals are returned in pasc_spg_result. ---- */
 int pasc_set_profiling(pasc_problem *p, int enable);
+
+/* ---- device copy bandwidth (STREAM-style copy kernel over two buffers of `bytes` each): the
+ * achievable HBM rate of this device, and the calibration kernel for traffic counters ---- */
+int pasc_device_copy_bandwidth(pasc_ctx *ctx, int64_t bytes, int32_t reps, double *gbps);
 
 /* ---- autotuner ----
  * Times the fused SPG pass for the problem's (K, precision, T) over the candidate

@@ -46,7 +46,7 @@ class SpgParams(C.Structure):
         ("sigma1", C.c_double), ("sigma2", C.c_double), ("alphainit", C.c_double),
         ("stop_mask", C.c_uint32), ("kkt_tol", C.c_double), ("check_every", C.c_int32),
         ("use_graph", C.c_int32), ("fp32_switch", C.c_double), ("stall_window", C.c_int32),
-        ("polish_maxit", C.c_int32),
+        ("polish_maxit", C.c_int32), ("kkt_every", C.c_int32),
     ]
 
 
@@ -121,6 +121,7 @@ _SIGS = {
                       C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int32)],
                      C.c_int),
     "pasc_set_profiling": ([vp, C.c_int], C.c_int),
+    "pasc_device_copy_bandwidth": ([vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)], C.c_int),
     "pasc_autotune": ([vp, C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                        C.POINTER(C.c_double)], C.c_int),
     "pasc_generate_markov": ([C.c_uint64, C.c_int64, C.c_int32, C.POINTER(C.c_double),

@@ -148,8 +148,9 @@ void pasc_spg_default_params(pasc_spg_params *p) {
     p->check_every = 1;
     p->use_graph = 0;
     p->fp32_switch = 0.0;
-    p->stall_window = 400;
+    p->stall_window = 200;
     p->polish_maxit = 0;
+    p->kkt_every = 1;
 }
 
 void pasc_ts_default_params(pasc_ts_params *p) {
@@ -165,6 +166,7 @@ static int check_params(const pasc_spg_params *p) {
     if (p->m < 1 || p->m > PASC_MAX_M) return fail(PASC_ERR_ARG, "m must be in [1, %d]", PASC_MAX_M);
     if (!(p->sigma2 >= p->sigma1)) return fail(PASC_ERR_ARG, "sigma2 < sigma1");
     if (!(p->alphainit > 0)) return fail(PASC_ERR_ARG, "alphainit must be > 0");
+    if (p->kkt_every < 1) return fail(PASC_ERR_ARG, "kkt_every must be >= 1");
     return PASC_OK;
 }
 
@@ -556,6 +558,14 @@ int pasc_adjoint(pasc_problem *p, const uint8_t *mask, double cg_tol, int32_t cg
     rc = engine_adjoint(p->e, 0, mask, cg_tol, cg_maxit, loss, dlogeps, dtheta, &it);
     if (cg_it) *cg_it = it;
     return rc;
+}
+
+int pasc_device_copy_bandwidth(pasc_ctx *ctx, int64_t bytes, int32_t reps, double *gbps) {
+    clear_error();
+    if (!ctx || !gbps || bytes < 16 || reps < 1) return fail(PASC_ERR_ARG, "copy_bandwidth: bad arguments");
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    return device_copy_bandwidth(&ctx->c, bytes, reps, gbps);
 }
 
 int pasc_set_profiling(pasc_problem *p, int enable) {

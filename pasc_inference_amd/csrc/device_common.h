@@ -31,17 +31,10 @@ template <> struct V16<double> {
     }
 };
 
-template <typename C> __device__ inline C cmax(C a, C b) { return a > b ? a : b; }
+// max via the hardware min/max instructions (one VALU op instead of a compare and a select)
+__device__ inline float cmax(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ inline double cmax(double a, double b) { return __builtin_fmax(a, b); }
 
-// Euclidean projection of one K-vector (in registers) onto the unit simplex
-// {y : sum y = 1, y >= 0}; replaces SimplexFeasibleSet_Local::project_sub
-// (simplex_local.h:176-249) with a sort-free threshold search:
-//   * exactly feasible input is returned unchanged (the reference's early exit);
-//   * tau is bracketed in [max(y) - 1, max(y)) (phi(tau) = sum max(y - tau, 0) - 1 changes sign
-//     there); NB bisection steps shrink the bracket, which matters for larger K;
-//   * the exact threshold is then the fixed point tau = (sum_{y > tau} y - 1) / |{y > tau}|
-//     iterated from the lower bracket end: the support only shrinks and at most K sweeps are
-//     needed, so the result is the same threshold as Chen & Ye's sorted search.
 template <typename C, int K>
 __device__ __forceinline__ C recip_small(int n) {
     // 1/n for n in [1, K] without a division (select chain over compile-time constants)
@@ -51,23 +44,59 @@ __device__ __forceinline__ C recip_small(int n) {
     return r;
 }
 
-// Returns the threshold tau (0 when the input is already feasible and left unchanged).
+// Threshold of the Euclidean projection onto the unit simplex {y : sum y = 1, y >= 0}, for
+// K <= 4: tau* = max over all non-empty subsets S of (sum_S y - 1)/|S|.  (With u sorted
+// descending, h(j) = (u_1 + .. + u_j - 1)/j is unimodal and peaks at the support size, so its
+// maximum is the Chen & Ye / Duchi threshold; any other subset of size j has a smaller sum than
+// the top-j prefix.)  Branch-free: no sort and no data-dependent loop.
+template <int K, typename C>
+__device__ __forceinline__ C simplex_tau_small(const C (&y)[K]) {
+    static_assert(K >= 1 && K <= 4, "closed form for K <= 4");
+    const C third = C(1) / C(3);
+    if constexpr (K == 1) {
+        return y[0] - C(1);
+    } else if constexpr (K == 2) {
+        const C t = cmax(y[0], y[1]) - C(1);
+        return cmax(t, C(0.5) * (y[0] + y[1] - C(1)));
+    } else if constexpr (K == 3) {
+        C t = cmax(cmax(y[0], y[1]), y[2]) - C(1);
+        const C s01 = y[0] + y[1], s02 = y[0] + y[2], s12 = y[1] + y[2];
+        t = cmax(t, C(0.5) * (cmax(cmax(s01, s02), s12) - C(1)));
+        return cmax(t, third * (s01 + y[2] - C(1)));
+    } else {
+        C t = cmax(cmax(y[0], y[1]), cmax(y[2], y[3])) - C(1);
+        const C s01 = y[0] + y[1], s02 = y[0] + y[2], s03 = y[0] + y[3];
+        const C s12 = y[1] + y[2], s13 = y[1] + y[3], s23 = y[2] + y[3];
+        const C p2 = cmax(cmax(cmax(s01, s02), cmax(s03, s12)), cmax(s13, s23));
+        t = cmax(t, C(0.5) * (p2 - C(1)));
+        const C s012 = s01 + y[2], s013 = s01 + y[3], s023 = s02 + y[3], s123 = s12 + y[3];
+        const C p3 = cmax(cmax(s012, s013), cmax(s023, s123));
+        t = cmax(t, third * (p3 - C(1)));
+        return cmax(t, C(0.25) * (s012 + y[3] - C(1)));
+    }
+}
+
+// Euclidean projection of one K-vector (in registers) onto the unit simplex; replaces
+// SimplexFeasibleSet_Local::project_sub (simplex_local.h:176-249) with a sort-free threshold:
+//   * K <= 4: the closed form above;
+//   * K > 4: tau is bracketed in [max(y) - 1, max(y)) (phi(tau) = sum max(y - tau, 0) - 1 changes
+//     sign there), NB bisection steps shrink the bracket, and the exact threshold is the fixed
+//     point tau = (sum_{y > tau} y - 1) / |{y > tau}| iterated from the lower bracket end (the
+//     support only shrinks, so at most K sweeps), the same threshold as Chen & Ye's search.
+// Returns tau; y is overwritten with max(y - tau, 0).
 template <int K, typename C>
 __device__ __forceinline__ C project_simplex(C (&y)[K]) {
-    C s = C(0);
-    bool neg = false;
+    if constexpr (K <= 4) {
+        const C tau = simplex_tau_small<K, C>(y);
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-        s += y[k];
-        neg |= (y[k] < C(0));
-    }
-    if (!neg && s == C(1)) return C(0);
-    C mx = y[0];
+        for (int k = 0; k < K; k++) y[k] = cmax(y[k] - tau, C(0));
+        return tau;
+    } else {
+        C mx = y[0];
 #pragma unroll
-    for (int k = 1; k < K; k++) mx = cmax(mx, y[k]);
-    C tau = mx - C(1);
-    constexpr int NB = K > 4 ? 4 : 0;
-    if (NB > 0) {
+        for (int k = 1; k < K; k++) mx = cmax(mx, y[k]);
+        C tau = mx - C(1);
+        constexpr int NB = 4;
         C hi = mx;
 #pragma unroll
         for (int b = 0; b < NB; b++) {
@@ -77,25 +106,25 @@ __device__ __forceinline__ C project_simplex(C (&y)[K]) {
             for (int k = 0; k < K; k++) phi += cmax(y[k] - mid, C(0));
             if (phi >= C(0)) tau = mid; else hi = mid;
         }
-    }
-    int cnt_prev = -1;
+        int cnt_prev = -1;
 #pragma unroll
-    for (int it = 0; it < K; it++) {
-        C sum = C(0);
-        int cnt = 0;
+        for (int it = 0; it < K; it++) {
+            C sum = C(0);
+            int cnt = 0;
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            const bool in = y[k] > tau;
-            sum += in ? y[k] : C(0);
-            cnt += in ? 1 : 0;
+            for (int k = 0; k < K; k++) {
+                const bool in = y[k] > tau;
+                sum += in ? y[k] : C(0);
+                cnt += in ? 1 : 0;
+            }
+            if (cnt == cnt_prev || cnt == 0) break;
+            cnt_prev = cnt;
+            tau = (sum - C(1)) * recip_small<C, K>(cnt);
         }
-        if (cnt == cnt_prev || cnt == 0) break;
-        cnt_prev = cnt;
-        tau = (sum - C(1)) * recip_small<C, K>(cnt);
-    }
 #pragma unroll
-    for (int k = 0; k < K; k++) y[k] = cmax(y[k] - tau, C(0));
-    return tau;
+        for (int k = 0; k < K; k++) y[k] = cmax(y[k] - tau, C(0));
+        return tau;
+    }
 }
 
 // Wavefront reductions (64 lanes).

@@ -301,6 +301,7 @@ int engine_get_gamma(Engine &e, int inst, double *g, int layout) {
 struct PhaseCfg {
     bool f64;
     StepParams sp;
+    int kkt_every;
 };
 
 static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<int> &grids,
@@ -320,7 +321,7 @@ static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<in
         const GroupView gv{g.d_ids, g.d_off, g.d_hint, g.n};
         int rc;
         if (e.stencil == STENCIL_TIME) {
-            PassArgs a{g0, g1, x, e.d_desc, e.d_state, e.d_part, gv};
+            PassArgs a{g0, g1, x, e.d_desc, e.d_state, e.d_part, gv, ph.kkt_every};
             rc = launch_pass_1d(ph.f64, g.K, a, grids[gi], e.block, ph.f64 ? e.chunk64() : e.chunk, s);
         } else {
             void *d0 = ph.f64 && e.precision == PASC_FP32_BULK ? e.d_dir64[0] : e.d_dir[0];
@@ -373,7 +374,7 @@ static int run_phase(Engine &e, const PhaseCfg &ph, const pasc_spg_params &p, in
     }
     std::vector<int> grids;
     for (auto &g : e.groups) {
-        int64_t cap = e.grid_cap > 0 ? e.grid_cap : (int64_t)std::max(e.ctx->n_cu, 1) * 32;
+        int64_t cap = e.grid_cap > 0 ? e.grid_cap : (int64_t)std::max(e.ctx->n_cu, 1) * 64;
         grids.push_back((int)std::max<int64_t>(1, std::min<int64_t>(g.max_tiles, cap)));
     }
     const int every = std::max(1, p.check_every);
@@ -477,9 +478,10 @@ static StepParams make_step(const pasc_spg_params &p, bool fp32_phase) {
     sp.kkt_tol = p.kkt_tol;
     sp.fp32_phase = fp32_phase ? 1 : 0;
     sp.stall_window = std::max(1, p.stall_window);
-    // hand over to FP64 once the FP32 residual is within 30x of the target (the FP32 floor of
+    // hand over to FP64 once the FP32 residual is within 100x of the target (the FP32 floor of
     // the residual is ~1e-5 at epsilon 100), or on stagnation
-    sp.fp32_switch = p.fp32_switch > 0.0 ? p.fp32_switch : 30.0 * p.kkt_tol;
+    sp.fp32_switch = p.fp32_switch > 0.0 ? p.fp32_switch : 100.0 * p.kkt_tol;
+    sp.fx_identity = fp32_phase ? 1 : 0;
     return sp;
 }
 
@@ -531,7 +533,7 @@ int engine_solve(Engine &e, const pasc_spg_params &p, pasc_spg_result *res_singl
     std::vector<int> it32(e.n_inst, 0), it64(e.n_inst, 0);
     int rc;
     if (!bulk32) {
-        PhaseCfg ph{true, make_step(p, false)};
+        PhaseCfg ph{true, make_step(p, false), p.kkt_every};
         rc = run_phase(e, ph, p, p.maxit + 2);
         if (rc) return rc;
         HIP_TRY(hipMemcpyAsync(st.data(), e.d_state, sizeof(InstState) * e.n_inst,
@@ -539,7 +541,7 @@ int engine_solve(Engine &e, const pasc_spg_params &p, pasc_spg_result *res_singl
         HIP_TRY(hipStreamSynchronize(s));
         for (int i = 0; i < e.n_inst; i++) { it64[i] = st[i].it; e.h_parity[i] = st[i].parity; }
     } else {
-        PhaseCfg ph{false, make_step(p, true)};
+        PhaseCfg ph{false, make_step(p, true), p.kkt_every};
         rc = run_phase(e, ph, p, p.maxit + 2);
         if (rc) return rc;
         HIP_TRY(hipMemcpyAsync(st.data(), e.d_state, sizeof(InstState) * e.n_inst,
@@ -574,7 +576,7 @@ int engine_solve(Engine &e, const pasc_spg_params &p, pasc_spg_result *res_singl
                                hipMemcpyHostToDevice, s));
         pasc_spg_params pp = p;
         pp.maxit = p.polish_maxit > 0 ? p.polish_maxit : p.maxit;
-        PhaseCfg ph2{true, make_step(pp, false)};
+        PhaseCfg ph2{true, make_step(pp, false), p.kkt_every};
         rc = run_phase(e, ph2, pp, pp.maxit + 2);
         if (rc) return rc;
         e.cur64 = true;
@@ -804,6 +806,47 @@ int engine_fill_uniform(Engine &e) {
 
 int engine_autotune(Engine &, int *, int *, double *) {
     return fail(PASC_ERR_STATE, "autotune: not available in this build");
+}
+
+__global__ __launch_bounds__(256) void pasc_copy_kernel(const float4 *__restrict__ a,
+                                                         float4 *__restrict__ b, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+int device_copy_bandwidth(Ctx *ctx, int64_t bytes, int reps, double *gbps) {
+    hipStream_t s = (hipStream_t)ctx->stream;
+    const int64_t n = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    HIP_TRY(hipMalloc(&a, 16 * (size_t)n));
+    hipError_t e = hipMalloc(&b, 16 * (size_t)n);
+    if (e != hipSuccess) {
+        (void)hipFree(a);
+        return fail(PASC_ERR_HIP, "hipMalloc: %s", hipGetErrorString(e));
+    }
+    HIP_TRY(hipMemsetAsync(a, 0, 16 * (size_t)n, s));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, (int64_t)ctx->n_cu * 16);
+    float best = 1e30f;
+    for (int r = 0; r < reps; r++) {
+        HIP_TRY(hipEventRecord(e0, s));
+        hipLaunchKernelGGL(pasc_copy_kernel, dim3(grid), dim3(256), 0, s, (const float4 *)a,
+                           (float4 *)b, n);
+        HIP_TRY(hipEventRecord(e1, s));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) best = ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    *gbps = 2.0 * 16.0 * (double)n / (best * 1e-3) / 1e9;
+    return PASC_OK;
 }
 
 // ------------------------------------------------------------------------------------------

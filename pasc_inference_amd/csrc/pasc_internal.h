@@ -64,6 +64,7 @@ struct StepParams {
     int32_t fp32_phase;   // 1: FP32 phase (stagnation / switch rules apply)
     int32_t stall_window;
     double fp32_switch;
+    int32_t fx_identity;  // f_i = f_{i-1} + beta (gd + beta dAd / 2) instead of the FP32 sum
 };
 
 enum StencilKind { STENCIL_TIME = 0, STENCIL_GRID = 1 };
@@ -149,6 +150,7 @@ int engine_adjoint(Engine &e, int inst, const uint8_t *mask, double cg_tol, int 
                    double *loss, double *dlogeps, double *dtheta, int *cg_it);
 int engine_autotune(Engine &e, int *block, int *chunk, double *us);
 
+int device_copy_bandwidth(Ctx *ctx, int64_t bytes, int reps, double *gbps);
 int kernel_project(Ctx *ctx, void *x, int64_t nblocks, int K, int dtype, int layout, int64_t ld);
 int kernel_matvec(Ctx *ctx, const void *x, void *y, int64_t T, int64_t R, int K, int64_t ld,
                   double alpha, int32_t grid_width, int dtype);

@@ -28,6 +28,7 @@ struct PassArgs {
     const InstState *state;
     double *part;
     GroupView grp;
+    int32_t kkt_every;  // evaluate the KKT residual on passes with it % kkt_every == 0
 };
 
 template <int CH>
@@ -79,9 +80,10 @@ __device__ __forceinline__ void exchange_edges(const C (&v)[K][CH], C *lds, C (&
 }
 
 template <typename C>
-__device__ __forceinline__ C lap2(C v, C vm, C vp, int64_t t, int64_t T) {
+__device__ __forceinline__ C lap2(C v, C vm, C vp, int64_t t, int64_t T, bool interior) {
     // row of A0 (blockgraphsparse.cpp:60-89) in difference form:
     // 2*(v_t - v_{t-1})[t>0] + 2*(v_t - v_{t+1})[t<T-1]
+    if (interior) return C(2) * ((v - vm) + (v - vp));
     C a = C(0);
     if (t > 0) a += v - vm;
     if (t < T - 1) a += v - vp;
@@ -125,13 +127,21 @@ __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int ch
         if (o && D.heldout) o = D.heldout[t] == 0;
         obs[c] = o;
     }
-    C th[K];
+    // data term of every owned step, computed once for the three levels
+    C Q[K][CH];
 #pragma unroll
-    for (int k = 0; k < K; k++) th[k] = C(D.theta[k]);
-    auto qterm = [&](int k, int c) -> C {
-        const C dx = X[c] - th[k];
-        return obs[c] ? bscale * dx * dx : C(0);
-    };
+    for (int k = 0; k < K; k++) {
+        const C th = C(D.theta[k]);
+#pragma unroll
+        for (int c = 0; c < CH; c++) {
+            const C dx = X[c] - th;
+            Q[k][c] = obs[c] ? bscale * dx * dx : C(0);
+        }
+    }
+    auto qterm = [&](int k, int c) -> C { return Q[k][c]; };
+    // every step of the chunk is interior (both time neighbours exist): no boundary selects
+    const bool interior = t0 >= 1 && t0 + CH <= T - 1;
+    const bool want_kkt = (st.it % (A.kkt_every > 0 ? A.kkt_every : 1)) == 0;
 
     // level 1: gamma_i = gamma_{i-1} + beta_{i-1} (P(gamma_{i-1} - alpha_{i-1} g_{i-1}) - gamma_{i-1})
     C G1[K][CH];
@@ -156,7 +166,7 @@ __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int ch
                 for (int k = 0; k < K; k++) {
                     const C vm = c > 0 ? G[k][c - 1] : gl[k];
                     const C vp = c < CH - 1 ? G[k][c + 1] : gr[k];
-                    const C g = alpha * lap2(G[k][c], vm, vp, t, T) + qterm(k, c);
+                    const C g = alpha * lap2(G[k][c], vm, vp, t, T, interior) + qterm(k, c);
                     y[k] = G[k][c] - a_apply * g;
                 }
                 project_simplex<K, C>(y);
@@ -178,7 +188,7 @@ __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int ch
             for (int k = 0; k < K; k++) {
                 const C vm = c > 0 ? G1[k][c - 1] : gl[k];
                 const C vp = c < CH - 1 ? G1[k][c + 1] : gr[k];
-                Gi[k][c] = alpha * lap2(G1[k][c], vm, vp, t, T) + qterm(k, c);
+                Gi[k][c] = alpha * lap2(G1[k][c], vm, vp, t, T, interior) + qterm(k, c);
                 y[k] = G1[k][c] - a_dir * Gi[k][c];
                 z[k] = y[k];
             }
@@ -214,22 +224,24 @@ __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int ch
                         const C d = Dd[k][c];
                         const C g = Gi[k][c];
                         ddt += (double)d * (double)d;
-                        if (t < T - 1) {
+                        if (interior || t < T - 1) {
                             // d'A d = alpha * sum over edges (t, t+1) of 2 (d_{t+1} - d_t)^2
                             const C dn = c < CH - 1 ? Dd[k][c + 1] : dr[k];
-                            const double e = (double)dn - (double)d;
-                            acc[1] += 2.0 * (double)alpha * e * e;
+                            const C e = dn - d;
+                            acc[1] += (double)(e * e);
                         }
                         acc[3] += 0.5 * ((double)g + (double)qterm(k, c)) * (double)G1[k][c];
                         z[k] = G1[k][c] - s_kkt * g;
                     }
                     acc[0] += ddt;
                     acc[2] -= inv_a * (ddt + (double)off[c]);
-                    project_simplex<K, C>(z);
+                    if (want_kkt) {
+                        project_simplex<K, C>(z);
 #pragma unroll
-                    for (int k = 0; k < K; k++) {
-                        const double e = fabs((double)G1[k][c] - (double)z[k]);
-                        acc[4] = (e > acc[4] || e != e) ? e : acc[4];
+                        for (int k = 0; k < K; k++) {
+                            const double e = fabs((double)G1[k][c] - (double)z[k]);
+                            acc[4] = (e > acc[4] || e != e) ? e : acc[4];
+                        }
                     }
                 }
             }
@@ -254,6 +266,8 @@ __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int ch
             }
         }
     }
+    acc[1] *= 2.0 * (double)alpha;
+    if (!want_kkt) acc[4] = -1.0;  // not evaluated on this pass
     block_reduce_part<NT>(acc, lds_red);
     if (threadIdx.x == 0) {
         double *o = A.part + (int64_t)(D.tile0 + chunk) * NPART;

@@ -19,7 +19,7 @@ __global__ __launch_bounds__(256) void spg_step(const InstDesc *desc, InstState 
     InstState &s = state[w];
     if (!s.active) return;
     const InstDesc &D = desc[w];
-    double v[NPART] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double v[NPART] = {0.0, 0.0, 0.0, 0.0, -1.0};  // residual -1: not evaluated
     for (int i = lane; i < D.ntiles; i += WAVE) {
         const double *p = part + (int64_t)(D.tile0 + i) * NPART;
 #pragma unroll
@@ -32,9 +32,19 @@ __global__ __launch_bounds__(256) void spg_step(const InstDesc *desc, InstState 
     v[NPART - 1] = wave_max(v[NPART - 1]);
     if (lane != 0) return;
 
-    const double dd = v[0], dAd = v[1], gd = v[2], f = v[3], kkt = v[4];
+    const double dd = v[0], dAd = v[1], gd = v[2];
+    // a negative residual marks a pass that did not evaluate it: keep the last value
+    const bool have_kkt = v[4] >= 0.0 || v[4] != v[4];
+    const double kkt = have_kkt ? v[4] : s.kkt;
+    // The FP32 phase takes f from the exact quadratic identity (the generic SPG's
+    // get_fx(fx_old, beta, gd, dAd), spgqpsolver.h:835-842): its increments are accurate, while
+    // the FP32 sum of 1/2 <g - b, gamma> is noisy at the level the line search compares.
+    const double f = (P.fx_identity && !s.first)
+                         ? s.f + s.beta_apply * (s.gd + 0.5 * s.beta_apply * s.dAd)
+                         : v[3];
     s.parity ^= 1;  // the pass wrote gamma_i into the other buffer
     s.kkt = kkt;
+    const bool kkt_fresh = have_kkt;
     bool stop = false;
     int reason = PASC_REASON_MAXIT;
     if (!(f == f) || !(dd == dd) || !(dAd == dAd) || !(gd == gd) || !(kkt == kkt)) {
@@ -45,7 +55,7 @@ __global__ __launch_bounds__(256) void spg_step(const InstDesc *desc, InstState 
         for (int i = 0; i < P.m; i++) s.fs[i] = f;
         s.fs_last = 0;
         s.it = 0;
-        if ((P.stop_mask & PASC_STOP_KKT) && kkt <= P.kkt_tol) { stop = true; reason = PASC_REASON_KKT; }
+        if ((P.stop_mask & PASC_STOP_KKT) && kkt_fresh && kkt <= P.kkt_tol) { stop = true; reason = PASC_REASON_KKT; }
     } else {
         s.fs_last = (s.fs_last + 1 >= P.m) ? 0 : s.fs_last + 1;
         s.fs[s.fs_last] = f;
@@ -55,15 +65,20 @@ __global__ __launch_bounds__(256) void spg_step(const InstDesc *desc, InstState 
         else if ((P.stop_mask & PASC_STOP_NORMGP_NORMB) && s.dd < P.eps * s.normb) { stop = true; reason = PASC_REASON_NORMGP_NORMB; }
         else if ((P.stop_mask & PASC_STOP_ANORMGP) && s.dAd < P.eps) { stop = true; reason = PASC_REASON_ANORMGP; }
         else if ((P.stop_mask & PASC_STOP_ANORMGP_NORMB) && s.dAd < P.eps * s.normb) { stop = true; reason = PASC_REASON_ANORMGP_NORMB; }
-        else if ((P.stop_mask & PASC_STOP_KKT) && kkt <= P.kkt_tol) { stop = true; reason = PASC_REASON_KKT; }
+        else if ((P.stop_mask & PASC_STOP_KKT) && kkt_fresh && kkt <= P.kkt_tol) { stop = true; reason = PASC_REASON_KKT; }
     }
     if (!stop && s.it >= P.maxit) { stop = true; reason = PASC_REASON_MAXIT; }
     if (!stop && dd == 0.0) { stop = true; reason = PASC_REASON_STATIONARY; }
     if (!stop && P.fp32_phase) {
         // FP32 phase ends at the switch threshold or when the KKT residual stagnates
-        if (kkt < 0.9 * s.best_kkt) { s.best_kkt = kkt; s.stall = 0; }
+        if (kkt_fresh && kkt < 0.9 * s.best_kkt) { s.best_kkt = kkt; s.stall = 0; }
         else s.stall += 1;
-        if ((P.fp32_switch > 0.0 && kkt <= P.fp32_switch) || s.stall >= P.stall_window) {
+        // FP32 storage cannot resolve the residual below about 8*eps*2^-24 (rounding of gamma
+        // amplified by the stencil), so the hand-over threshold never goes below a fraction
+        // of that floor
+        const double floor32 = 2.4e-7 * D.alpha * D.s_kkt;
+        const double sw = P.fp32_switch > floor32 ? P.fp32_switch : floor32;
+        if ((kkt_fresh && kkt <= sw) || s.stall >= P.stall_window) {
             stop = true;
             reason = PASC_REASON_MAXIT;
         }

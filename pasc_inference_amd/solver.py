@@ -28,6 +28,12 @@ class Context:
     def synchronize(self):
         check(lib().pasc_ctx_synchronize(self.h))
 
+    def copy_bandwidth(self, nbytes: int = 1 << 30, reps: int = 5) -> float:
+        """Best GB/s (read + write) of a plain device copy kernel over nbytes."""
+        g = C.c_double()
+        check(lib().pasc_device_copy_bandwidth(self.h, nbytes, reps, C.byref(g)))
+        return g.value
+
     def set_stream(self, stream_ptr: int | None):
         check(lib().pasc_ctx_set_stream(self.h, C.c_void_p(stream_ptr) if stream_ptr else None))
 

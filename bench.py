@@ -99,6 +99,15 @@ def run_c2(args, ctx):
     ctx.synchronize()
     t64 = time.perf_counter() - t0
     log(args, f"plain fp64 solve: {t64:.4f}s it={r64.it} kkt={r64.kkt:.2e}")
+    # FP64 path with the graph-captured iteration loop (same kernels, no per-iteration poll)
+    P64.set_gamma_uniform()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    r64g = P64.gamma_solve(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt,
+                           check_every=32, use_graph=0 if args.no_graph else 1, kkt_every=4)
+    ctx.synchronize()
+    t64g = time.perf_counter() - t0
+    log(args, f"fp64 graph solve: {t64g:.4f}s it={r64g.it} kkt={r64g.kkt:.2e}")
     # FP64 reference solve (tight KKT) for the error statement
     P64.set_profiling(True)
     P64.set_gamma_uniform()
@@ -133,7 +142,7 @@ def run_c2(args, ctx):
             traffic = json.load(open(tf)).get("bytes_per_launch")
         except Exception:
             traffic = None
-    c2 = dict(times=times, res=res, t64=t64, r64=r64, err=err, avg_pass=avg_pass,
+    c2 = dict(times=times, res=res, t64=t64, r64=r64, t64g=t64g, r64g=r64g, err=err, avg_pass=avg_pass,
               avg_pass64=avg_pass64, achieved=achieved, own_bytes=own_bytes,
               alg_bytes=alg_bytes, traffic=traffic, x=x, theta=theta, T=T, K=K,
               ref_pass64=(rref.pass_time_fp64_s / max(rref.passes_fp64, 1)))
@@ -280,6 +289,8 @@ def main():
         "iterations": {"fp32": r.it_fp32, "fp64_polish": r.it_fp64,
                        "plain_fp64": c2["r64"].it},
         "plain_fp64_time_to_kkt_s": c2["t64"],
+        "fp64_graph_time_to_kkt_s": c2["t64g"],
+        "fp64_graph_iterations": c2["r64g"].it,
         "speedup_vs_plain_fp64": c2["t64"] / float(np.mean(c2["times"])),
         "errors_vs_fp64_reference_solve": c2["err"],
         "roofline": {

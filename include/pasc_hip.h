@@ -97,8 +97,8 @@ typedef struct {
     int32_t check_every;  /* iterations between host convergence polls (default 1) */
     int32_t use_graph;    /* capture check_every iterations in a hipGraph (default 0) */
     double fp32_switch;   /* FP32-bulk: hand over to FP64 once the FP32 KKT <= this
-                             (default 0 = 100 * kkt_tol; never below the FP32
-                             floor 2.4e-7 * epsilon of the residual) */
+                             (default 0 = 100 * kkt_tol; never below the level
+                             1e-5 * epsilon where FP32 storage stalls SPG) */
     int32_t stall_window; /* FP32-bulk: iterations without a 10% improvement of the best KKT
                              residual that end the FP32 phase (default 200) */
     int32_t polish_maxit; /* FP32-bulk: maximum FP64 polish iterations (default maxit) */

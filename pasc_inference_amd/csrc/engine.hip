@@ -143,7 +143,7 @@ int engine_alloc(Engine &e, Ctx *ctx, int precision, int stencil, int32_t grid_w
     }
     HIP_TRY(hipMalloc(&e.d_desc, sizeof(InstDesc) * std::max(e.n_inst, 1)));
     HIP_TRY(hipMalloc(&e.d_state, sizeof(InstState) * std::max(e.n_inst, 1)));
-    HIP_TRY(hipMalloc(&e.d_nactive, sizeof(int32_t)));
+    HIP_TRY(hipMalloc(&e.d_nactive, 2 * sizeof(int32_t)));  // [0] active count, [1] dirty flag
     // pinned poll area: [0] active instances, [1 + g] active tiles of K group g
     HIP_TRY(hipHostMalloc((void **)&e.h_nactive, sizeof(int32_t) * (1 + PASC_MAX_K),
                           hipHostMallocDefault));
@@ -338,9 +338,8 @@ static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<in
         HIP_TRY(hipEventRecord(b, s));
         ev->push_back(b);
     }
-    const int wpb = 256 / WAVE;
-    hipLaunchKernelGGL(spg_step, dim3((e.n_inst + wpb - 1) / wpb), dim3(256), 0, s, e.d_desc,
-                       e.d_state, e.d_part, e.n_inst, ph.sp);
+    hipLaunchKernelGGL(spg_step, dim3(e.n_inst), dim3(256), 0, s, e.d_desc, e.d_state, e.d_part,
+                       e.n_inst, ph.sp, e.d_nactive + 1);
     ScanArgs sa{};
     sa.n_groups = (int)e.groups.size();
     for (size_t gi = 0; gi < e.groups.size(); gi++)
@@ -348,6 +347,7 @@ static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<in
     sa.desc = e.d_desc;
     sa.state = e.d_state;
     sa.nactive = e.d_nactive;
+    sa.dirty = e.d_nactive + 1;
     hipLaunchKernelGGL(spg_scan, dim3(1), dim3(1024), 0, s, sa);
     HIP_TRY(hipGetLastError());
     return PASC_OK;
@@ -369,6 +369,9 @@ static int run_phase(Engine &e, const PhaseCfg &ph, const pasc_spg_params &p, in
         sa.desc = e.d_desc;
         sa.state = e.d_state;
         sa.nactive = e.d_nactive;
+        sa.dirty = e.d_nactive + 1;
+        // force the initial rebuild (any non-zero dirty flag)
+        HIP_TRY(hipMemsetAsync(e.d_nactive + 1, 1, sizeof(int32_t), s));
         hipLaunchKernelGGL(spg_scan, dim3(1), dim3(1024), 0, s, sa);
         HIP_TRY(hipGetLastError());
     }

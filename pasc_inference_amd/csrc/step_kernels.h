@@ -12,25 +12,25 @@ namespace pasc {
 // (:216) and the stopping tests (:260-274) plus the KKT test.
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void spg_step(const InstDesc *desc, InstState *state,
-                                                 const double *part, int n_inst, StepParams P) {
-    const int w = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-    const int lane = threadIdx.x & (WAVE - 1);
+                                                 const double *part, int n_inst, StepParams P,
+                                                 int32_t *dirty) {
+    // one workgroup per instance: a single instance of T = 1e7 owns ~1e4 partial slots
+    __shared__ double lds[NPART * 4];
+    const int w = blockIdx.x;
     if (w >= n_inst) return;
     InstState &s = state[w];
     if (!s.active) return;
     const InstDesc &D = desc[w];
     double v[NPART] = {0.0, 0.0, 0.0, 0.0, -1.0};  // residual -1: not evaluated
-    for (int i = lane; i < D.ntiles; i += WAVE) {
+    for (int i = threadIdx.x; i < D.ntiles; i += blockDim.x) {
         const double *p = part + (int64_t)(D.tile0 + i) * NPART;
 #pragma unroll
         for (int q = 0; q < NPART - 1; q++) v[q] += p[q];
         const double m = p[NPART - 1];
         v[NPART - 1] = (m > v[NPART - 1] || m != m) ? m : v[NPART - 1];
     }
-#pragma unroll
-    for (int q = 0; q < NPART - 1; q++) v[q] = wave_sum(v[q]);
-    v[NPART - 1] = wave_max(v[NPART - 1]);
-    if (lane != 0) return;
+    block_reduce_part<256>(v, lds);
+    if (threadIdx.x != 0) return;
 
     const double dd = v[0], dAd = v[1], gd = v[2];
     // a negative residual marks a pass that did not evaluate it: keep the last value
@@ -73,10 +73,11 @@ __global__ __launch_bounds__(256) void spg_step(const InstDesc *desc, InstState 
         // FP32 phase ends at the switch threshold or when the KKT residual stagnates
         if (kkt_fresh && kkt < 0.9 * s.best_kkt) { s.best_kkt = kkt; s.stall = 0; }
         else s.stall += 1;
-        // FP32 storage cannot resolve the residual below about 8*eps*2^-24 (rounding of gamma
-        // amplified by the stencil), so the hand-over threshold never goes below a fraction
-        // of that floor
-        const double floor32 = 2.4e-7 * D.alpha * D.s_kkt;
+        // With gamma rounded to FP32 after every update, SPG stalls once the residual nears
+        // ~1e-5 * epsilon (per-sample epsilon = alpha * s_kkt): the stencil amplifies the
+        // rounding of gamma by up to 8 epsilon and the BB steps degrade well above that floor.
+        // The FP32 phase therefore hands over no later than at that level.
+        const double floor32 = 1e-5 * D.alpha * D.s_kkt;
         const double sw = P.fp32_switch > floor32 ? P.fp32_switch : floor32;
         if ((kkt_fresh && kkt <= sw) || s.stall >= P.stall_window) {
             stop = true;
@@ -92,6 +93,7 @@ __global__ __launch_bounds__(256) void spg_step(const InstDesc *desc, InstState 
         s.active = 0;
         s.reason = reason;
         s.first = 0;
+        *dirty = 1;  // the active-tile enumeration must be rebuilt
         return;
     }
     double fmax = s.fs[0];
@@ -138,11 +140,13 @@ struct ScanArgs {
     const InstDesc *desc;
     const InstState *state;
     int32_t *nactive;
+    int32_t *dirty;  // set by the step kernel when an instance stopped; cleared here
 };
 
 __global__ __launch_bounds__(1024) void spg_scan(ScanArgs A) {
     __shared__ int32_t sh[1024];
     __shared__ int32_t carry;
+    if (*A.dirty == 0) return;  // nothing stopped since the last rebuild
     int active_total = 0;
     for (int gi = 0; gi < A.n_groups; gi++) {
         const ScanGroup G = A.g[gi];
@@ -172,12 +176,17 @@ __global__ __launch_bounds__(1024) void spg_scan(ScanArgs A) {
         }
         if (threadIdx.x == 0) G.off[G.n] = carry;
         __syncthreads();
-        // hints: bucket b (tiles [64b, 64b+64)) starts at the position whose range holds 64b
-        for (int a = threadIdx.x; a < G.n; a += 1024) {
-            const int lo = G.off[a], hi = G.off[a + 1];
-            if (hi > lo) {
-                for (int b = (lo + 63) >> 6; (b << 6) < hi; b++) G.hint[b] = a;
+        // hints: bucket b (tiles [64b, 64b+64)) starts at the position whose range holds 64b;
+        // one thread per bucket, binary search over the scan
+        const int total = G.off[G.n];
+        for (int b = threadIdx.x; (b << 6) < total; b += 1024) {
+            const int j = b << 6;
+            int lo = 0, hi = G.n;  // invariant: off[lo] <= j < off[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (G.off[mid] <= j) lo = mid; else hi = mid;
             }
+            G.hint[b] = lo;
         }
         __syncthreads();
     }
@@ -188,7 +197,10 @@ __global__ __launch_bounds__(1024) void spg_scan(ScanArgs A) {
         if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *A.nactive = red[0];
+    if (threadIdx.x == 0) {
+        *A.nactive = red[0];
+        *A.dirty = 0;
+    }
 }
 
 }  // namespace pasc

@@ -72,8 +72,12 @@ static int engine_retile(Engine &e, int tl) {
     e.cur_tile = tl;
     int slots = 0;
     for (auto &d : e.h_desc) {
+        // at most ~2048 partial slots per instance: enough workgroups to fill the device, few
+        // enough that the step kernel reduces them quickly
+        const int64_t raw = (d.T + tl - 1) / tl;
+        d.msub = (int32_t)std::max<int64_t>(1, (raw + 2047) / 2048);
         d.tile0 = slots;
-        d.ntiles = (int32_t)((d.T + tl - 1) / tl);
+        d.ntiles = (int32_t)((raw + d.msub - 1) / d.msub);
         slots += d.ntiles;
     }
     if (slots != e.total_slots || !e.d_part) {
@@ -338,8 +342,8 @@ static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<in
         HIP_TRY(hipEventRecord(b, s));
         ev->push_back(b);
     }
-    hipLaunchKernelGGL(spg_step, dim3(e.n_inst), dim3(256), 0, s, e.d_desc, e.d_state, e.d_part,
-                       e.n_inst, ph.sp, e.d_nactive + 1);
+    hipLaunchKernelGGL(spg_step, dim3(std::min(e.n_inst, std::max(e.ctx->n_cu, 1) * 4)), dim3(256),
+                       0, s, e.d_desc, e.d_state, e.d_part, e.n_inst, ph.sp, e.d_nactive + 1);
     ScanArgs sa{};
     sa.n_groups = (int)e.groups.size();
     for (size_t gi = 0; gi < e.groups.size(); gi++)

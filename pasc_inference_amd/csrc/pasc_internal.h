@@ -23,8 +23,9 @@ struct InstDesc {
     int64_t xoff;     // offset of the instance's data in the data buffer, in elements
     int32_t K;
     int32_t tile0;    // first partial slot of this instance
-    int32_t ntiles;   // partial slots used per pass
+    int32_t ntiles;   // partial slots used per pass (each covers msub consecutive tiles)
     int32_t xdim;
+    int32_t msub;     // tiles per partial slot
     double alpha;     // A = alpha * A0
     double bscale;    // b = -bscale * ||x - theta_k||^2
     double s_kkt;     // step of the natural KKT residual

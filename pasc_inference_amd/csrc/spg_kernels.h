@@ -92,7 +92,7 @@ __device__ __forceinline__ C lap2(C v, C vm, C vp, int64_t t, int64_t T, bool in
 
 template <typename S, typename C, int K, int CH, int NT>
 __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int chunk,
-                                             double *lds_red, C *lds_x) {
+                                             C *lds_x, double (&acc)[NPART]) {
     constexpr int HT = halo_threads<CH>();
     constexpr int TILE = tile_steps<NT, CH>();
     const InstDesc &D = A.desc[inst];
@@ -207,7 +207,7 @@ __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int ch
         }
     }
     // level 3: reductions on the owned steps, store gamma_i
-    double acc[NPART] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double dad = 0.0, kkt = acc[4];
     {
         C dl[K], dr[K];
         exchange_edges<C, K, CH, NT>(Dd, lds_x, dl, dr);
@@ -228,7 +228,7 @@ __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int ch
                             // d'A d = alpha * sum over edges (t, t+1) of 2 (d_{t+1} - d_t)^2
                             const C dn = c < CH - 1 ? Dd[k][c + 1] : dr[k];
                             const C e = dn - d;
-                            acc[1] += (double)(e * e);
+                            dad += (double)(e * e);
                         }
                         acc[3] += 0.5 * ((double)g + (double)qterm(k, c)) * (double)G1[k][c];
                         z[k] = G1[k][c] - s_kkt * g;
@@ -240,7 +240,7 @@ __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int ch
 #pragma unroll
                         for (int k = 0; k < K; k++) {
                             const double e = fabs((double)G1[k][c] - (double)z[k]);
-                            acc[4] = (e > acc[4] || e != e) ? e : acc[4];
+                            kkt = (e > kkt || e != e) ? e : kkt;
                         }
                     }
                 }
@@ -266,25 +266,33 @@ __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int ch
             }
         }
     }
-    acc[1] *= 2.0 * (double)alpha;
-    if (!want_kkt) acc[4] = -1.0;  // not evaluated on this pass
-    block_reduce_part<NT>(acc, lds_red);
-    if (threadIdx.x == 0) {
-        double *o = A.part + (int64_t)(D.tile0 + chunk) * NPART;
-#pragma unroll
-        for (int i = 0; i < NPART; i++) o[i] = acc[i];
-    }
+    acc[1] += 2.0 * (double)alpha * dad;
+    acc[4] = want_kkt ? kkt : -1.0;  // -1: not evaluated on this pass
 }
 
 template <typename S, typename C, int K, int CH, int NT>
 __global__ __launch_bounds__(NT) void spg_pass_1d(PassArgs A) {
     __shared__ double lds_red[NPART * (NT / WAVE)];
     __shared__ C lds_x[2 * K * NT];
+    constexpr int TILE = tile_steps<NT, CH>();
     const int total = A.grp.off[A.grp.n];
     for (int j = blockIdx.x; j < total; j += gridDim.x) {
-        int inst, chunk;
-        find_tile(A.grp, j, inst, chunk);
-        pass_tile_1d<S, C, K, CH, NT>(A, inst, chunk, lds_red, lds_x);
+        int inst, slot;
+        find_tile(A.grp, j, inst, slot);
+        const InstDesc &D = A.desc[inst];
+        // one partial slot covers msub consecutive tiles of the instance
+        double acc[NPART] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int m = 0; m < D.msub; m++) {
+            const int tile = slot * D.msub + m;
+            if ((int64_t)tile * TILE >= D.T) break;
+            pass_tile_1d<S, C, K, CH, NT>(A, inst, tile, lds_x, acc);
+        }
+        block_reduce_part<NT>(acc, lds_red);
+        if (threadIdx.x == 0) {
+            double *o = A.part + (int64_t)(D.tile0 + slot) * NPART;
+#pragma unroll
+            for (int i = 0; i < NPART; i++) o[i] = acc[i];
+        }
     }
 }
 
@@ -349,18 +357,20 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_dir(GridArgs A) {
     const int total = A.grp.off[A.grp.n];
     const bool is_f32 = sizeof(S) == 4;
     for (int j = blockIdx.x; j < total; j += gridDim.x) {
-        int inst, chunk;
-        find_tile(A.grp, j, inst, chunk);
+        int inst, slot;
+        find_tile(A.grp, j, inst, slot);
         const InstDesc &D = A.desc[inst];
         const InstState &st = A.state[inst];
         const int64_t R = D.T;
         const int64_t W = A.width;
-        const int64_t r = (int64_t)chunk * NT + threadIdx.x;
         const int par = st.parity;
+        double acc[NPART] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      for (int m = 0; m < D.msub; m++) {
+        const int64_t r = (int64_t)(slot * D.msub + m) * NT + threadIdx.x;
+        if ((int64_t)(slot * D.msub + m) * NT >= R) break;
         const S *src = reinterpret_cast<const S *>(par ? A.gamma1 : A.gamma0) + D.goff;
         S *dst = reinterpret_cast<S *>(par ? A.gamma0 : A.gamma1) + D.goff;
         S *dr = reinterpret_cast<S *>(par ? A.dir0 : A.dir1) + D.goff;
-        double acc[NPART] = {0.0, 0.0, 0.0, 0.0, 0.0};
         if (r < R) {
             const int64_t row = r / W, col = r - row * W;
             C g0[K];
@@ -406,9 +416,10 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_dir(GridArgs A) {
             // <g, d> in the sign-safe form of the projected-gradient step (see pass_tile_1d)
             acc[2] -= (ddt + off) / (double)a_dir;
         }
+      }
         block_reduce_part<NT>(acc, lds);
         if (threadIdx.x == 0) {
-            double *o = A.part + (int64_t)(D.tile0 + chunk) * NPART;
+            double *o = A.part + (int64_t)(D.tile0 + slot) * NPART;
             o[2] = acc[2];
             o[3] = acc[3];
             o[4] = acc[4];
@@ -421,15 +432,17 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_ad(GridArgs A) {
     __shared__ double lds[NPART * (NT / WAVE)];
     const int total = A.grp.off[A.grp.n];
     for (int j = blockIdx.x; j < total; j += gridDim.x) {
-        int inst, chunk;
-        find_tile(A.grp, j, inst, chunk);
+        int inst, slot;
+        find_tile(A.grp, j, inst, slot);
         const InstDesc &D = A.desc[inst];
         const int64_t R = D.T;
         const int64_t W = A.width;
-        const int64_t r = (int64_t)chunk * NT + threadIdx.x;
         // d_i was written by spg_pass_grid_dir into the buffer opposite to the parity
         const S *dr = reinterpret_cast<const S *>(A.state[inst].parity ? A.dir0 : A.dir1) + D.goff;
         double acc[NPART] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      for (int m = 0; m < D.msub; m++) {
+        const int64_t r = (int64_t)(slot * D.msub + m) * NT + threadIdx.x;
+        if ((int64_t)(slot * D.msub + m) * NT >= R) break;
         if (r < R) {
             const int64_t row = r / W, col = r - row * W;
             const int64_t nb[4] = {row > 0 ? r - W : -1, col > 0 ? r - 1 : -1,
@@ -452,9 +465,10 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_ad(GridArgs A) {
                 }
             }
         }
+      }
         block_reduce_part<NT>(acc, lds);
         if (threadIdx.x == 0) {
-            double *o = A.part + (int64_t)(D.tile0 + chunk) * NPART;
+            double *o = A.part + (int64_t)(D.tile0 + slot) * NPART;
             o[0] = acc[0];
             o[1] = acc[1];
         }

@@ -6,32 +6,13 @@
 namespace pasc {
 
 // ---------------------------------------------------------------------------------------------
-// Step kernel: one wavefront per instance.  Reduces the instance's tile partials in a fixed
-// order (deterministic), then runs the scalar part of the reference iteration on lane 0:
-// spg_fs (spg_fs.cpp:17-63), the generalised Armijo step (spgqpsolver.cpp:182-195), the BB step
-// (:216) and the stopping tests (:260-274) plus the KKT test.
+// Step: the scalar part of the reference iteration for one instance -- spg_fs
+// (spg_fs.cpp:17-63), the generalised Armijo step (spgqpsolver.cpp:182-195), the BB step (:216)
+// and the stopping tests (:260-274) plus the KKT test -- run by thread 0 of the workgroup that
+// reduced the instance's partial slots.
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void spg_step(const InstDesc *desc, InstState *state,
-                                                 const double *part, int n_inst, StepParams P,
-                                                 int32_t *dirty) {
-    // one workgroup per instance: a single instance of T = 1e7 owns ~1e4 partial slots
-    __shared__ double lds[NPART * 4];
-    const int w = blockIdx.x;
-    if (w >= n_inst) return;
-    InstState &s = state[w];
-    if (!s.active) return;
-    const InstDesc &D = desc[w];
-    double v[NPART] = {0.0, 0.0, 0.0, 0.0, -1.0};  // residual -1: not evaluated
-    for (int i = threadIdx.x; i < D.ntiles; i += blockDim.x) {
-        const double *p = part + (int64_t)(D.tile0 + i) * NPART;
-#pragma unroll
-        for (int q = 0; q < NPART - 1; q++) v[q] += p[q];
-        const double m = p[NPART - 1];
-        v[NPART - 1] = (m > v[NPART - 1] || m != m) ? m : v[NPART - 1];
-    }
-    block_reduce_part<256>(v, lds);
-    if (threadIdx.x != 0) return;
-
+__device__ inline void step_scalar(const InstDesc &D, InstState &s, const double (&v)[NPART],
+                                   const StepParams &P, int32_t *dirty) {
     const double dd = v[0], dAd = v[1], gd = v[2];
     // a negative residual marks a pass that did not evaluate it: keep the last value
     const bool have_kkt = v[4] >= 0.0 || v[4] != v[4];
@@ -118,6 +99,31 @@ __global__ __launch_bounds__(256) void spg_step(const InstDesc *desc, InstState 
     s.alpha_apply = s.alpha_dir;
     s.alpha_dir = alpha_next;
     s.first = 0;
+}
+
+
+// Step kernel: workgroups stride over the instances; each reduces one active instance's partial
+// slots in a fixed order (deterministic) and runs step_scalar.
+__global__ __launch_bounds__(256) void spg_step(const InstDesc *desc, InstState *state,
+                                                 const double *part, int n_inst, StepParams P,
+                                                 int32_t *dirty) {
+    // one workgroup per instance: a single instance of T = 1e7 owns ~1e4 partial slots
+    __shared__ double lds[NPART * 4];
+    for (int w = blockIdx.x; w < n_inst; w += gridDim.x) {
+    InstState &s = state[w];
+    if (!s.active) continue;
+    const InstDesc &D = desc[w];
+    double v[NPART] = {0.0, 0.0, 0.0, 0.0, -1.0};  // residual -1: not evaluated
+    for (int i = threadIdx.x; i < D.ntiles; i += blockDim.x) {
+        const double *p = part + (int64_t)(D.tile0 + i) * NPART;
+#pragma unroll
+        for (int q = 0; q < NPART - 1; q++) v[q] += p[q];
+        const double m = p[NPART - 1];
+        v[NPART - 1] = (m > v[NPART - 1] || m != m) ? m : v[NPART - 1];
+    }
+    block_reduce_part<256>(v, lds);
+    if (threadIdx.x == 0) step_scalar(D, s, v, P, dirty);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -33,6 +33,7 @@ def parse():
     ap.add_argument("--kkt", type=float, default=1e-6)
     ap.add_argument("--no-scan", action="store_true")
     ap.add_argument("--scan-series", type=int, default=256)
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--plain-sample", type=int, default=32,
                     help="instances of the scan timed on the plain FP64 per-solve path")
     ap.add_argument("--scan-maxit", type=int, default=20000,
@@ -68,6 +69,12 @@ def run_c2(args, ctx):
     n = T * K
     P32 = Problem(ctx, x, K, precision=capi.FP32_BULK, epssqr=eps)
     P32.set_theta(theta)
+    tune = {}
+    if not args.no_autotune:
+        P32.set_gamma_uniform()
+        tune["fp32"] = P32.autotune()
+        log(args, f"autotune fp32: block={tune['fp32'][0]} chunk={tune['fp32'][1]} "
+                  f"{tune['fp32'][2]:.1f} us/pass")
     params = dict(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt, check_every=32,
                   use_graph=0 if args.no_graph else 1, kkt_every=4)
     for _ in range(args.warmup):
@@ -101,7 +108,12 @@ def run_c2(args, ctx):
     ctx.synchronize()
     t64 = time.perf_counter() - t0
     log(args, f"plain fp64 solve: {t64:.4f}s it={r64.it} kkt={r64.kkt:.2e}")
-    # FP64 path with the graph-captured iteration loop (same kernels, no per-iteration poll)
+    # FP64 path with the graph-captured iteration loop and the autotuned launch shape
+    if not args.no_autotune:
+        P64.set_gamma_uniform()
+        tune["fp64"] = P64.autotune()
+        log(args, f"autotune fp64: block={tune['fp64'][0]} chunk={tune['fp64'][1]} "
+                  f"{tune['fp64'][2]:.1f} us/pass")
     P64.set_gamma_uniform()
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -144,7 +156,7 @@ def run_c2(args, ctx):
             traffic = json.load(open(tf)).get("bytes_per_launch")
         except Exception:
             traffic = None
-    c2 = dict(times=times, res=res, t64=t64, r64=r64, t64g=t64g, r64g=r64g, err=err, avg_pass=avg_pass,
+    c2 = dict(times=times, res=res, t64=t64, r64=r64, t64g=t64g, r64g=r64g, err=err, tune=tune, avg_pass=avg_pass,
               avg_pass64=avg_pass64, achieved=achieved, own_bytes=own_bytes,
               alg_bytes=alg_bytes, traffic=traffic, x=x, theta=theta, T=T, K=K,
               ref_pass64=(rref.pass_time_fp64_s / max(rref.passes_fp64, 1)))
@@ -307,6 +319,8 @@ def main():
         "plain_fp64_time_to_kkt_s": c2["t64"],
         "fp64_graph_time_to_kkt_s": c2["t64g"],
         "fp64_graph_iterations": c2["r64g"].it,
+        "autotune": {k: {"block": v[0], "chunk": v[1], "us_per_pass": v[2]}
+                     for k, v in c2["tune"].items()},
         "speedup_vs_plain_fp64": c2["t64"] / best,
         # whole solve against the roofline, 6*n*s algorithmic bytes per SPG iteration
         "solve_roofline_frac": solve_bytes / best / (HBM_PEAK_GBS * 1e9),

@@ -584,7 +584,23 @@ int pasc_autotune(pasc_problem *p, const char *cache_path, int32_t *block, int32
     if (rc) return rc;
     int bl = 0, ch = 0;
     double us = 0;
-    rc = engine_autotune(p->e, &bl, &ch, &us);
+    // the choice is cached per context, keyed by device, K, precision and log2(T)
+    char key[128];
+    snprintf(key, sizeof(key), "%s/K%d/p%d/T2^%d", p->ctx->c.arch.c_str(), p->K, p->precision,
+             (int)std::floor(std::log2((double)p->T)));
+    bool hit = false;
+    for (auto &kv : p->ctx->c.tune)
+        if (kv.first == key) {
+            bl = kv.second.first;
+            ch = kv.second.second;
+            hit = true;
+        }
+    if (hit) {
+        rc = engine_set_shape(p->e, bl, ch);
+    } else {
+        rc = engine_autotune(p->e, &bl, &ch, &us);
+        if (rc == PASC_OK) p->ctx->c.tune.push_back({key, {bl, ch}});
+    }
     if (rc) return rc;
     if (block) *block = bl;
     if (chunk) *chunk = ch;
@@ -592,8 +608,8 @@ int pasc_autotune(pasc_problem *p, const char *cache_path, int32_t *block, int32
     if (cache_path) {
         FILE *f = fopen(cache_path, "a");
         if (f) {
-            fprintf(f, "{\"arch\": \"%s\", \"K\": %d, \"precision\": %d, \"T\": %lld, \"block\": %d, \"chunk\": %d, \"us_per_pass\": %.3f}\n",
-                    p->ctx->c.arch.c_str(), p->K, p->precision, (long long)p->T, bl, ch, us);
+            fprintf(f, "{\"key\": \"%s\", \"T\": %lld, \"block\": %d, \"chunk\": %d, \"us_per_pass\": %.3f, \"cached\": %d}\n",
+                    key, (long long)p->T, bl, ch, us, hit ? 1 : 0);
             fclose(f);
         }
     }

@@ -811,8 +811,76 @@ int engine_fill_uniform(Engine &e) {
     return PASC_OK;
 }
 
-int engine_autotune(Engine &, int *, int *, double *) {
-    return fail(PASC_ERR_STATE, "autotune: not available in this build");
+int engine_set_shape(Engine &e, int block, int chunk) {
+    const bool f32 = e.precision == PASC_FP32_BULK;
+    const bool ok_chunk = f32 ? (chunk == 4 || chunk == 8) : (chunk == 2 || chunk == 4);
+    if (!(block == 128 || block == 256 || block == 512) || !ok_chunk)
+        return fail(PASC_ERR_ARG, "no fused pass for block %d chunk %d", block, chunk);
+    std::vector<std::vector<double>> saved(e.n_inst);
+    for (int i = 0; i < e.n_inst; i++) {
+        int rc = download_rows(e, i, saved[i]);
+        if (rc) return rc;
+    }
+    e.block = block;
+    e.chunk = chunk;
+    int rc = engine_retile(e, tile_len(e, !f32));
+    for (int i = 0; i < e.n_inst && rc == PASC_OK; i++) rc = upload_rows(e, i, saved[i]);
+    return rc;
+}
+
+// Self-written autotuner: times the fused pass of this problem for every compiled
+// (threads per block, time steps per thread) pair -- 20 passes each from the current iterate,
+// hipEvents around every launch -- and keeps the fastest.  The iterate is restored afterwards.
+int engine_autotune(Engine &e, int *block, int *chunk, double *us) {
+    if (e.stencil != STENCIL_TIME) return fail(PASC_ERR_ARG, "autotune: 1-D time graph only");
+    std::vector<std::vector<double>> saved(e.n_inst);
+    for (int i = 0; i < e.n_inst; i++) {
+        int rc = download_rows(e, i, saved[i]);
+        if (rc) return rc;
+    }
+    const bool f32 = e.precision == PASC_FP32_BULK;
+    const int chunks[2] = {f32 ? 4 : 2, f32 ? 8 : 4};
+    const int blocks[3] = {128, 256, 512};
+    pasc_spg_params p;
+    pasc_spg_default_params(&p);
+    p.maxit = 20;
+    p.stop_mask = 0;
+    p.polish_maxit = 1;
+    const bool prof = e.profile;
+    e.profile = true;
+    double best = 1e300;
+    int bb = e.block, bc = e.chunk;
+    for (int ci = 0; ci < 2; ci++) {
+        for (int bi = 0; bi < 3; bi++) {
+            e.block = blocks[bi];
+            e.chunk = chunks[ci];
+            int rc = engine_retile(e, tile_len(e, !f32));
+            for (int i = 0; i < e.n_inst && rc == PASC_OK; i++) rc = upload_rows(e, i, saved[i]);
+            pasc_spg_result r;
+            if (rc == PASC_OK) rc = engine_solve(e, p, &r, nullptr, nullptr);
+            if (rc) {
+                e.profile = prof;
+                return rc;
+            }
+            const double t = f32 ? e.prof_ms[0] / std::max(e.prof_n[0], 1)
+                                 : e.prof_ms[1] / std::max(e.prof_n[1], 1);
+            if (t < best) {
+                best = t;
+                bb = e.block;
+                bc = e.chunk;
+            }
+        }
+    }
+    e.profile = prof;
+    e.block = bb;
+    e.chunk = bc;
+    int rc = engine_retile(e, tile_len(e, !f32));
+    for (int i = 0; i < e.n_inst && rc == PASC_OK; i++) rc = upload_rows(e, i, saved[i]);
+    if (rc) return rc;
+    *block = bb;
+    *chunk = bc;
+    *us = best * 1e3;
+    return PASC_OK;
 }
 
 __global__ __launch_bounds__(256) void pasc_copy_kernel(const float4 *__restrict__ a,

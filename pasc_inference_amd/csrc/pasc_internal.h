@@ -150,6 +150,7 @@ int engine_abserr(Engine &e, int inst, const double *solution, double *abserr);
 int engine_adjoint(Engine &e, int inst, const uint8_t *mask, double cg_tol, int cg_maxit,
                    double *loss, double *dlogeps, double *dtheta, int *cg_it);
 int engine_autotune(Engine &e, int *block, int *chunk, double *us);
+int engine_set_shape(Engine &e, int block, int chunk);
 
 int device_copy_bandwidth(Ctx *ctx, int64_t bytes, int reps, double *gbps);
 int kernel_project(Ctx *ctx, void *x, int64_t nblocks, int K, int dtype, int layout, int64_t ld);

@@ -215,3 +215,28 @@ def test_batch_matches_individual_solves(ctx, oracle):
                 assert abs(fo - ro.fx) <= 1e-8 * abs(ro.fx)
                 assert label_agreement(g, ref, K) >= 0.999
         B.close()
+
+
+def test_autotune_keeps_iterate_and_results(ctx, oracle):
+    """The autotuner times every launch shape from the current iterate, restores it, and the
+    chosen shape solves to the same minimiser."""
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Problem
+    T, K, eps = 30000, 3, 20.0
+    x, _ = _signal(21, T, [0.0, 1.0, 3.0], 1e-3, 0.9)
+    theta = np.array([0.0, 1.0, 3.0])
+    ref, rr, b = _oracle_min(oracle, x, theta, T, K, eps)
+    for prec in [capi.FP64, capi.FP32_BULK]:
+        P = Problem(ctx, x, K, precision=prec, epssqr=eps)
+        P.set_theta(theta)
+        g0 = oracle.project_simplex(np.random.default_rng(1).random(T * K), K)
+        P.set_gamma(g0)
+        block, chunk, us = P.autotune()
+        print(f"prec={prec}: block={block} chunk={chunk} {us:.1f} us/pass")
+        assert block in (128, 256, 512) and us > 0
+        np.testing.assert_allclose(P.get_gamma(), g0, atol=1e-7 if prec else 0)
+        assert P.autotune()[:2] == (block, chunk)  # cached per context
+        P.set_gamma_uniform()
+        r = P.gamma_solve(maxit=100000, stop_mask=capi.STOP_KKT, kkt_tol=1e-9)
+        g = P.get_gamma()
+        assert r.kkt <= 1e-9 and np.max(np.abs(g - ref)) <= 1e-6

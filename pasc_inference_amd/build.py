@@ -60,9 +60,29 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
     os.replace(tmp, LIB)
+    build_host(verbose)
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+HOST_DRIVER = os.path.join(HERE, "host", "test_signal1d")
+
+
+def build_host(verbose: bool = False) -> str:
+    """The C++ host layer's driver (host/examples/test_signal1d.cpp), linked to the library."""
+    gxx = shutil.which("g++") or "g++"
+    root = os.path.dirname(HERE)
+    cmd = [gxx, "-std=c++17", "-O2", "-Wall", "-Wextra",
+           "-I", os.path.join(root, "include"), "-I", os.path.join(HERE, "host", "include"),
+           os.path.join(HERE, "host", "examples", "test_signal1d.cpp"),
+           "-L", HERE, "-lpasc_hip", "-Wl,-rpath,$ORIGIN/..", "-o", HOST_DRIVER]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host driver build failed:\n{r.stderr[-6000:]}")
+    if verbose and r.stderr.strip():
+        print(r.stderr)
+    return HOST_DRIVER
 
 
 if __name__ == "__main__":

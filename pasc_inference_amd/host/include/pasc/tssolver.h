@@ -1,0 +1,124 @@
+// TSSolver<HipVector>: the outer theta/gamma alternation (include/general/solver/tssolver.h,
+// solve() :491-715, set_solution_theta :777-790, defaults :13-15).
+#pragma once
+
+#include "common.h"
+#include "model.h"
+
+namespace pascinference {
+namespace solver {
+
+template <class VectorBase> class TSSolver;
+
+template <> class TSSolver<HipVector> : public GeneralSolver {
+  public:
+    TSSolver(data::TSData<HipVector> &tsdata, model::TSModel<HipVector> &model,
+             const Options &opt = Options())
+        : tsdata_(&tsdata), model_(&model) {
+        opt.set_option_value("tssolver_maxit", &maxit, 300);
+        opt.set_option_value("tssolver_eps", &eps, 1e-6);
+        opt.set_option_value("tssolver_debugmode", &debugmode, 0);
+        model_->initialize_gammasolver(&gammasolver_);
+        model_->initialize_thetasolver(&thetasolver_);
+    }
+    ~TSSolver() override {
+        model_->finalize_gammasolver(&gammasolver_);
+        model_->finalize_thetasolver(&thetasolver_);
+    }
+    TSSolver(const TSSolver &) = delete;
+    TSSolver &operator=(const TSSolver &) = delete;
+
+    std::string get_name() const override { return "TSSolver<HipVector>"; }
+
+    // set_solution_theta: theta is given and never updated
+    void set_solution_theta(const std::vector<double> &theta) {
+        tsdata_->set_thetavector(theta);
+        thetasolved_ = true;
+    }
+    // start with a gamma step from the theta already set (the first theta update of the
+    // reference would collapse a uniform gamma0 to equal thetas)
+    void set_skip_first_theta(bool s) { skip_first_theta_ = s; }
+
+    void solve() override {
+        timer_solve.start();
+        double L = std::numeric_limits<double>::max(), L_old;
+        int it;
+        for (it = 0; it < maxit; it++) {
+            if (!thetasolved_ && !(it == 0 && skip_first_theta_)) {
+                timer_theta_update.start();
+                model_->updatebeforesolve_thetasolver(thetasolver_);
+                timer_theta_update.stop();
+                thetasolver_->solve();
+                model_->updateaftersolve_thetasolver(thetasolver_);
+            }
+            timer_gamma_update.start();
+            model_->updatebeforesolve_gammasolver(gammasolver_);
+            timer_gamma_update.stop();
+            timer_gamma_solve.start();
+            gammasolver_->solve();
+            timer_gamma_solve.stop();
+            model_->updateaftersolve_gammasolver(gammasolver_);
+
+            L_old = L;
+            L = model_->get_L(gammasolver_, thetasolver_);
+            deltaL_ = std::fabs(L - L_old);
+            if (debugmode > 0)
+                std::cout << " outer loop: it = " << it << ", L = " << L << ", |L - L_old| = " << deltaL_
+                          << std::endl;
+            if (L - L_old > 0) {
+                std::cout << "ERROR: objective function increased" << std::endl;  // tssolver.h:637-639
+                increased_++;
+            }
+            if (deltaL_ < eps) break;
+        }
+        it_last_ = it;
+        it_sum_ += it;
+        L_ = L;
+        tsdata_->set_aic(model_->get_aic(L));
+        timer_solve.stop();
+    }
+
+    double get_L() const { return L_; }
+    double get_deltaL() const { return deltaL_; }
+    int get_it() const override { return it_sum_; }
+    int get_increased() const { return increased_; }
+    GeneralSolver *get_gammasolver() const { return gammasolver_; }
+    GeneralSolver *get_thetasolver() const { return thetasolver_; }
+
+    void printstatus(std::ostream &o) const override {
+        o << get_name() << std::endl;
+        o << " - it:      " << it_last_ << std::endl;
+        o << " - L:       " << L_ << std::endl;
+        o << " - deltaL:  " << deltaL_ << std::endl;
+        gammasolver_->printstatus(o);
+    }
+    void printtimer(std::ostream &o) const override {
+        o << get_name() << std::endl;
+        o << " - t_solve =        " << timer_solve.get_value_sum() << std::endl;
+        o << " - t_gamma_update = " << timer_gamma_update.get_value_sum() << std::endl;
+        o << " - t_gamma_solve =  " << timer_gamma_solve.get_value_sum() << std::endl;
+        o << " - t_theta_update = " << timer_theta_update.get_value_sum() << std::endl;
+        gammasolver_->printtimer(o);
+    }
+    void printshort(std::ostringstream &header, std::ostringstream &values) const override {
+        header << "TSSolver it, TSSolver L, TSSolver t all, t gamma solve, ";
+        values << it_last_ << ", " << L_ << ", " << timer_solve.get_value_last() << ", "
+               << timer_gamma_solve.get_value_sum() << ", ";
+        gammasolver_->printshort(header, values);
+    }
+
+  private:
+    data::TSData<HipVector> *tsdata_;
+    model::TSModel<HipVector> *model_;
+    GeneralSolver *gammasolver_ = nullptr;
+    GeneralSolver *thetasolver_ = nullptr;
+    bool thetasolved_ = false;
+    bool skip_first_theta_ = false;
+    double L_ = std::numeric_limits<double>::max();
+    double deltaL_ = std::numeric_limits<double>::max();
+    int it_last_ = 0, it_sum_ = 0, increased_ = 0;
+    Timer timer_solve, timer_gamma_solve, timer_gamma_update, timer_theta_update;
+};
+
+}  // namespace solver
+}  // namespace pascinference

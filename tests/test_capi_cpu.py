@@ -111,3 +111,20 @@ def test_null_arguments_are_rejected():
     tp = capi.TsParams()
     L.pasc_ts_default_params(C.byref(tp))
     assert (tp.maxit, tp.eps) == (300, 1e-6)  # tssolver.h:13-14
+
+
+def test_cpp_host_driver_is_built_and_fails_loudly_without_device():
+    import os
+    import subprocess
+    drv = os.path.join(os.path.dirname(capi.HERE), "pasc_inference_amd", "host", "test_signal1d")
+    assert os.path.exists(drv)
+    r = subprocess.run([drv], capture_output=True, text=True)
+    assert r.returncode == 1 and "test_epssqr has to be set" in r.stdout
+    try:
+        import torch
+        if torch.cuda.device_count() > 0:
+            return
+    except Exception:
+        pass
+    r = subprocess.run([drv, "--test_epssqr=1"], capture_output=True, text=True)
+    assert r.returncode == 2 and "pasc error" in r.stderr

@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--profile-run", action="store_true",
                     help="only a fixed number of fused passes of the T=1e7 solve in each "
                          "precision plus a copy kernel (for rocprofv3 traces and counters)")
+    ap.add_argument("--tune-cache", default=None,
+                    help="autotuner cache file: launch shapes recorded there are reused, new ones appended")
     ap.add_argument("--verbose", action="store_true")
     return ap.parse_args()
 
@@ -72,7 +74,7 @@ def run_c2(args, ctx):
     tune = {}
     if not args.no_autotune:
         P32.set_gamma_uniform()
-        tune["fp32"] = P32.autotune()
+        tune["fp32"] = P32.autotune(args.tune_cache)
         log(args, f"autotune fp32: block={tune['fp32'][0]} chunk={tune['fp32'][1]} "
                   f"{tune['fp32'][2]:.1f} us/pass")
     params = dict(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt, check_every=32,
@@ -111,7 +113,7 @@ def run_c2(args, ctx):
     # FP64 path with the graph-captured iteration loop and the autotuned launch shape
     if not args.no_autotune:
         P64.set_gamma_uniform()
-        tune["fp64"] = P64.autotune()
+        tune["fp64"] = P64.autotune(args.tune_cache)
         log(args, f"autotune fp64: block={tune['fp64'][0]} chunk={tune['fp64'][1]} "
                   f"{tune['fp64'][2]:.1f} us/pass")
     P64.set_gamma_uniform()
@@ -249,7 +251,9 @@ def run_cpu_baseline(args, c2):
 
 
 def profile_run(args):
-    """Fixed work for profilers: 64 fused FP32 passes, 64 FP64 passes, a 1 GiB copy kernel."""
+    """Fixed work for profilers: 64 fused FP32 passes, 64 FP64 passes, a 1 GiB copy kernel.
+
+    With --tune-cache the passes use the launch shapes an earlier bench run recorded there."""
     from pasc_inference_amd import capi
     from pasc_inference_amd.solver import Context, Problem
     x, _, theta, eps = c2_workload(args)
@@ -260,8 +264,10 @@ def profile_run(args):
             P = Problem(ctx, x, theta.size, precision=prec, epssqr=eps)
             P.set_theta(theta)
             P.set_gamma_uniform()
+            if args.tune_cache:
+                P.autotune(args.tune_cache)
             P.set_profiling(True)
-            r = P.gamma_solve(maxit=64, stop_mask=0, polish_maxit=1)
+            r = P.gamma_solve(maxit=64, stop_mask=0, polish_maxit=1, check_every=16)
             P.close()
             t = r.pass_time_fp32_s if prec == capi.FP32_BULK else r.pass_time_fp64_s
             n = r.passes_fp32 if prec == capi.FP32_BULK else r.passes_fp64

@@ -172,6 +172,14 @@ int pasc_set_epssqr(pasc_problem *p, double epssqr);
 void pasc_spg_default_params(pasc_spg_params *p);
 int pasc_gamma_solve(pasc_problem *p, const pasc_spg_params *params, pasc_spg_result *res);
 
+/* ---- explicit linear term: the b of QPData (qpdata->get_b(), spgqpsolver.cpp:60,104-106) ----
+ * With b set, pasc_gamma_solve minimises 1/2 g'Ag - b'g over the simplices with this b in place
+ * of the data term -(x - theta_k)^2/T formed from x and theta (graphh1fem.cpp:236-289); A is
+ * still alpha*A0 with alpha from pasc_set_epssqr.  b has T*R*K entries in the given layout;
+ * b == NULL returns to the data term.  pasc_ts_solve and pasc_adjoint reject a problem with an
+ * explicit b. */
+int pasc_set_linear_term(pasc_problem *p, const double *b, int layout);
+
 /* ---- the theta step: updatebeforesolve_thetasolver (graphh1fem.cpp:305-396), FP64 sums ---- */
 int pasc_theta_update(pasc_problem *p);
 
@@ -252,7 +260,9 @@ int pasc_device_copy_bandwidth(pasc_ctx *ctx, int64_t bytes, int32_t reps, doubl
 /* ---- autotuner ----
  * Times the fused SPG pass for the problem's (K, precision, T) over the candidate
  * (threads per block, time steps per thread) pairs and keeps the fastest for later solves.
- * The choice is cached per context and written to cache_path when it is non-NULL. */
+ * The choice is cached per context.  With a non-NULL cache_path, a choice recorded there for
+ * the same key (device, K, precision, log2 T) by an earlier process is reused without timing,
+ * and every call appends its choice to the file (one JSON object per line). */
 int pasc_autotune(pasc_problem *p, const char *cache_path, int32_t *block, int32_t *chunk,
                   double *us_per_pass);
 

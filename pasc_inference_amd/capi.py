@@ -22,6 +22,7 @@ STOP_DIFFF, STOP_NORMGP, STOP_NORMGP_NORMB, STOP_ANORMGP, STOP_ANORMGP_NORMB, ST
     1, 2, 4, 8, 16, 32)
 REASONS = ["maxit", "difff", "normgp", "normgp_normb", "Anormgp", "Anormgp_normb", "kkt",
            "stationary", "numeric"]
+REASON_KKT = REASONS.index("kkt")
 MAX_K = 8
 
 
@@ -100,6 +101,7 @@ _SIGS = {
     "pasc_spg_default_params": ([C.POINTER(SpgParams)], None),
     "pasc_gamma_solve": ([vp, C.POINTER(SpgParams), C.POINTER(SpgResult)], C.c_int),
     "pasc_theta_update": ([vp], C.c_int),
+    "pasc_set_linear_term": ([vp, C.POINTER(C.c_double), C.c_int], C.c_int),
     "pasc_ts_default_params": ([C.POINTER(TsParams)], None),
     "pasc_ts_solve": ([vp, C.POINTER(TsParams), C.POINTER(SpgParams), C.POINTER(TsResult)],
                       C.c_int),

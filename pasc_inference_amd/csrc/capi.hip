@@ -52,6 +52,7 @@ struct pasc_problem {
     bool theta_set = false;
     bool gamma_set = false;
     bool solved = false;
+    bool b_set = false;  // explicit linear term instead of the data term
     uint8_t *d_heldout = nullptr;
     std::vector<uint8_t> h_heldout;
     std::vector<double> h_x;  // host copy of the data, [T][R][xdim] (for abserr etc.)
@@ -326,7 +327,8 @@ int pasc_gamma_solve(pasc_problem *p, const pasc_spg_params *params, pasc_spg_re
     if (!p) return fail(PASC_ERR_ARG, "gamma_solve: NULL problem");
     int rc = check_params(params);
     if (rc) return rc;
-    if (!p->theta_set) return fail(PASC_ERR_STATE, "gamma_solve: theta was never set");
+    if (!p->theta_set && !p->b_set)
+        return fail(PASC_ERR_STATE, "gamma_solve: neither theta nor an explicit b was set");
     rc = use_device(p->ctx);
     if (rc) return rc;
     if (!p->gamma_set) {
@@ -339,6 +341,25 @@ int pasc_gamma_solve(pasc_problem *p, const pasc_spg_params *params, pasc_spg_re
     rc = engine_solve(p->e, *params, &r, nullptr, nullptr);
     if (res) *res = r;
     if (rc == PASC_OK) p->solved = true;
+    return rc;
+}
+
+int pasc_set_linear_term(pasc_problem *p, const double *b, int layout) {
+    clear_error();
+    if (!p) return fail(PASC_ERR_ARG, "set_linear_term: NULL problem");
+    if (layout != PASC_LAYOUT_TMAJOR && layout != PASC_LAYOUT_KMAJOR) return fail(PASC_ERR_ARG, "bad layout");
+    if (b) {
+        const int64_t n = (p->grid_width > 0 ? p->R : p->T) * (int64_t)p->K;
+        for (int64_t i = 0; i < n; i++)
+            if (!std::isfinite(b[i])) return fail(PASC_ERR_ARG, "set_linear_term: b[%lld] is not finite", (long long)i);
+    }
+    int rc = use_device(p->ctx);
+    if (rc) return rc;
+    rc = engine_set_linear_term(p->e, 0, b, layout);
+    if (rc == PASC_OK) {
+        p->b_set = b != nullptr;
+        p->solved = false;
+    }
     return rc;
 }
 
@@ -357,6 +378,7 @@ int pasc_ts_solve(pasc_problem *p, const pasc_ts_params *tp, const pasc_spg_para
                   pasc_ts_result *res) {
     clear_error();
     if (!p || !tp) return fail(PASC_ERR_ARG, "ts_solve: NULL argument");
+    if (p->b_set) return fail(PASC_ERR_STATE, "ts_solve: the problem has an explicit b (theta would not enter it)");
     int rc = check_params(sp);
     if (rc) return rc;
     rc = use_device(p->ctx);
@@ -552,6 +574,7 @@ int pasc_adjoint(pasc_problem *p, const uint8_t *mask, double cg_tol, int32_t cg
     if (!p || !mask || !loss || !dlogeps || !dtheta) return fail(PASC_ERR_ARG, "adjoint: NULL argument");
     if (p->xdim != 1 || p->grid_width > 0) return fail(PASC_ERR_ARG, "adjoint: 1-D signals (xdim == 1) only");
     if (!p->solved) return fail(PASC_ERR_STATE, "adjoint: solve the gamma problem first");
+    if (p->b_set) return fail(PASC_ERR_STATE, "adjoint: the problem has an explicit b");
     int rc = use_device(p->ctx);
     if (rc) return rc;
     int it = 0;
@@ -595,6 +618,27 @@ int pasc_autotune(pasc_problem *p, const char *cache_path, int32_t *block, int32
             ch = kv.second.second;
             hit = true;
         }
+    if (!hit && cache_path) {
+        // a cache file written by an earlier process: the last line with this key wins
+        FILE *f = fopen(cache_path, "r");
+        if (f) {
+            char line[512], want[160];
+            snprintf(want, sizeof(want), "\"key\": \"%s\"", key);
+            while (fgets(line, sizeof(line), f)) {
+                if (!strstr(line, want)) continue;
+                const char *pb = strstr(line, "\"block\": ");
+                const char *pc = strstr(line, "\"chunk\": ");
+                int b2 = 0, c2 = 0;
+                if (pb && pc && sscanf(pb + 9, "%d", &b2) == 1 && sscanf(pc + 9, "%d", &c2) == 1) {
+                    bl = b2;
+                    ch = c2;
+                    hit = true;
+                }
+            }
+            fclose(f);
+            if (hit) p->ctx->c.tune.push_back({key, {bl, ch}});
+        }
+    }
     if (hit) {
         rc = engine_set_shape(p->e, bl, ch);
     } else {

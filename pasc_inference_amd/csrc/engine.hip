@@ -181,6 +181,9 @@ void engine_free(Engine &e) {
     }
     if (e.d_x64) (void)hipFree(e.d_x64);
     if (e.d_x32) (void)hipFree(e.d_x32);
+    if (e.d_b64) (void)hipFree(e.d_b64);
+    if (e.d_b32) (void)hipFree(e.d_b32);
+    e.d_b64 = e.d_b32 = nullptr;
     if (e.d_desc) (void)hipFree(e.d_desc);
     if (e.d_state) (void)hipFree(e.d_state);
     if (e.d_part) (void)hipFree(e.d_part);
@@ -286,6 +289,43 @@ int engine_set_gamma(Engine &e, int inst, const double *g, int layout) {
     return upload_rows(e, inst, rows);
 }
 
+// Explicit linear term of one instance (the QP's b as the reference's QPData holds it): stored
+// k-major like gamma, FP64 plus an FP32 copy for the FP32 phase.  b_host == nullptr returns the
+// instance to the data term formed from x and theta.
+int engine_set_linear_term(Engine &e, int inst, const double *b_host, int layout) {
+    InstDesc &d = e.h_desc[inst];
+    if (!b_host) {
+        d.b64 = nullptr;
+        d.b32 = nullptr;
+        return engine_update_desc(e);
+    }
+    if (!e.d_b64) {
+        HIP_TRY(hipMalloc(&e.d_b64, 8 * (size_t)e.state_elems));
+        HIP_TRY(hipMemsetAsync(e.d_b64, 0, 8 * (size_t)e.state_elems, S_(e)));
+        if (e.precision == PASC_FP32_BULK) {
+            HIP_TRY(hipMalloc(&e.d_b32, 4 * (size_t)e.state_elems));
+            HIP_TRY(hipMemsetAsync(e.d_b32, 0, 4 * (size_t)e.state_elems, S_(e)));
+        }
+    }
+    const int64_t n = (int64_t)d.K * d.ld;
+    std::vector<double> rows((size_t)n, 0.0);
+    for (int k = 0; k < d.K; k++)
+        for (int64_t t = 0; t < d.T; t++)
+            rows[k * d.ld + t] = layout == PASC_LAYOUT_TMAJOR ? b_host[t * d.K + k] : b_host[k * d.T + t];
+    double *b64 = (double *)e.d_b64 + d.goff;
+    HIP_TRY(hipMemcpyAsync(b64, rows.data(), 8 * (size_t)n, hipMemcpyHostToDevice, S_(e)));
+    float *b32 = nullptr;
+    if (e.d_b32) {
+        b32 = (float *)e.d_b32 + d.goff;
+        hipLaunchKernelGGL((convert_kernel<double, float>), dim3((unsigned)((n + 255) / 256)),
+                           dim3(256), 0, S_(e), (const double *)b64, b32, n);
+        HIP_TRY(hipGetLastError());
+    }
+    d.b64 = b64;
+    d.b32 = b32;
+    return engine_update_desc(e);  // synchronises: rows may go out of scope
+}
+
 int engine_get_gamma(Engine &e, int inst, double *g, int layout) {
     const InstDesc &d = e.h_desc[inst];
     std::vector<double> rows;
@@ -325,7 +365,8 @@ static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<in
         const GroupView gv{g.d_ids, g.d_off, g.d_hint, g.n};
         int rc;
         if (e.stencil == STENCIL_TIME) {
-            PassArgs a{g0, g1, x, e.d_desc, e.d_state, e.d_part, gv, ph.kkt_every};
+            PassArgs a{g0, g1, x, e.d_desc, e.d_state, e.d_part, gv, ph.kkt_every,
+                       ph.sp.fx_identity ? 0 : 1};
             rc = launch_pass_1d(ph.f64, g.K, a, grids[gi], e.block, ph.f64 ? e.chunk64() : e.chunk, s);
         } else {
             void *d0 = ph.f64 && e.precision == PASC_FP32_BULK ? e.d_dir64[0] : e.d_dir[0];

@@ -30,6 +30,10 @@ struct InstDesc {
     double bscale;    // b = -bscale * ||x - theta_k||^2
     double s_kkt;     // step of the natural KKT residual
     const uint8_t *heldout;  // device mask or nullptr
+    // explicit linear term b (k-major rows of stride ld, like gamma) or nullptr: the data term
+    // of the iteration is then -b instead of bscale*||x - theta_k||^2
+    const double *b64;
+    const float *b32;
     double theta[PASC_MAX_K * 3];  // theta[k*xdim + c], xdim <= 3
 };
 
@@ -92,6 +96,8 @@ struct Engine {
     void *d_dir64[2] = {nullptr, nullptr};
     void *d_x64 = nullptr;                    // data, FP64, rows of stride ld
     void *d_x32 = nullptr;                    // data, FP32 copy (FP32-bulk only)
+    void *d_b64 = nullptr;                    // explicit linear term (single instance), FP64
+    void *d_b32 = nullptr;                    //   and its FP32 copy (FP32-bulk only)
     InstDesc *d_desc = nullptr;
     InstState *d_state = nullptr;
     double *d_part = nullptr;                 // total_slots * NPART
@@ -135,6 +141,7 @@ int engine_alloc(Engine &e, Ctx *ctx, int precision, int stencil, int32_t grid_w
                  const std::vector<InstDesc> &descs, int64_t data_elems);
 void engine_free(Engine &e);
 int engine_upload_data(Engine &e, const double *x_host, int64_t offset, int64_t n);
+int engine_set_linear_term(Engine &e, int inst, const double *b_host, int layout);
 int engine_set_uniform(Engine &e, int inst);
 int engine_fill_uniform(Engine &e);
 int engine_set_gamma(Engine &e, int inst, const double *gamma_host, int layout);

@@ -29,6 +29,7 @@ struct PassArgs {
     double *part;
     GroupView grp;
     int32_t kkt_every;  // evaluate the KKT residual on passes with it % kkt_every == 0
+    int32_t need_f;     // 0: f comes from the quadratic identity (only the first pass sums it)
 };
 
 template <int CH>
@@ -38,8 +39,8 @@ template <int NT, int CH>
 constexpr int tile_steps() { return (NT - 2 * halo_threads<CH>()) * CH; }
 
 // Loads CH consecutive steps [t0, t0+CH) of one row (t0 a multiple of the vector width);
-// steps outside [0, T) read as 0.
-template <typename S, typename C, int CH>
+// steps outside [0, T) read as 0.  INT: the caller guarantees [t0, t0+CH) inside [0, T).
+template <typename S, typename C, int CH, bool INT = false>
 __device__ __forceinline__ void load_chunk(const S *__restrict__ row, int64_t t0, int64_t T,
                                            int64_t ld, C (&v)[CH]) {
     using VT = V16<S>;
@@ -48,7 +49,11 @@ __device__ __forceinline__ void load_chunk(const S *__restrict__ row, int64_t t0
 #pragma unroll
     for (int q = 0; q < CH / VW; q++) {
         const int64_t t = t0 + q * VW;
-        if (t >= 0 && t < ld) {
+        if (INT) {
+            const typename VT::type vec = *reinterpret_cast<const typename VT::type *>(row + t);
+#pragma unroll
+            for (int i = 0; i < VW; i++) v[q * VW + i] = C(VT::get(vec, i));
+        } else if (t >= 0 && t < ld) {
             const typename VT::type vec = *reinterpret_cast<const typename VT::type *>(row + t);
 #pragma unroll
             for (int i = 0; i < VW; i++) v[q * VW + i] = (t + i < T) ? C(VT::get(vec, i)) : C(0);
@@ -90,7 +95,10 @@ __device__ __forceinline__ C lap2(C v, C vm, C vp, int64_t t, int64_t T, bool in
     return C(2) * a;
 }
 
-template <typename S, typename C, int K, int CH, int NT>
+// One tile of the fused pass.  INT: every step of the tile, halos included, lies in [1, T-2]
+// (the common case; no bounds or boundary-row tests).  Per-thread partials accumulate in the
+// compute type C over the thread's owned steps and enter the FP64 slot sums once per tile.
+template <typename S, typename C, int K, int CH, int NT, bool INT>
 __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int chunk,
                                              C *lds_x, double (&acc)[NPART]) {
     constexpr int HT = halo_threads<CH>();
@@ -115,32 +123,46 @@ __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int ch
     const C b_apply = C(st.beta_apply);
     const C a_dir = C(st.alpha_dir);
     const bool first = st.first != 0;
+    const bool need_f = A.need_f || first;
 
-    // data term q = bscale*(x - theta_k)^2 = -b (zero when held out or outside [0,T))
-    C X[CH];
-    load_chunk<S, C, CH>(xr, t0, T, ld, X);
-    bool obs[CH];
-#pragma unroll
-    for (int c = 0; c < CH; c++) {
-        const int64_t t = t0 + c;
-        bool o = (t >= 0 && t < T);
-        if (o && D.heldout) o = D.heldout[t] == 0;
-        obs[c] = o;
-    }
-    // data term of every owned step, computed once for the three levels
+    // data term q = -b of every step of the chunk, formed once for the two levels that use it:
+    // bscale*(x - theta_k)^2 (zero when held out or outside [0,T)), or read from an explicit b
     C Q[K][CH];
+    if (D.b64) {
+        const S *br = sizeof(S) == 4 ? reinterpret_cast<const S *>(D.b32)
+                                     : reinterpret_cast<const S *>(D.b64);
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-        const C th = C(D.theta[k]);
+        for (int k = 0; k < K; k++) {
+            load_chunk<S, C, CH, INT>(br + k * ld, t0, T, ld, Q[k]);
+#pragma unroll
+            for (int c = 0; c < CH; c++) Q[k][c] = -Q[k][c];
+        }
+    } else {
+        C X[CH];
+        load_chunk<S, C, CH, INT>(xr, t0, T, ld, X);
+        bool obs[CH];
 #pragma unroll
         for (int c = 0; c < CH; c++) {
-            const C dx = X[c] - th;
-            Q[k][c] = obs[c] ? bscale * dx * dx : C(0);
+            const int64_t t = t0 + c;
+            obs[c] = INT || (t >= 0 && t < T);
+        }
+        if (D.heldout) {
+#pragma unroll
+            for (int c = 0; c < CH; c++)
+                if (obs[c]) obs[c] = D.heldout[t0 + c] == 0;
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const C th = C(D.theta[k]);
+#pragma unroll
+            for (int c = 0; c < CH; c++) {
+                const C dx = X[c] - th;
+                Q[k][c] = obs[c] ? bscale * dx * dx : C(0);
+            }
         }
     }
-    auto qterm = [&](int k, int c) -> C { return Q[k][c]; };
     // every step of the chunk is interior (both time neighbours exist): no boundary selects
-    const bool interior = t0 >= 1 && t0 + CH <= T - 1;
+    const bool interior = INT || (t0 >= 1 && t0 + CH <= T - 1);
     const bool want_kkt = (st.it % (A.kkt_every > 0 ? A.kkt_every : 1)) == 0;
 
     // level 1: gamma_i = gamma_{i-1} + beta_{i-1} (P(gamma_{i-1} - alpha_{i-1} g_{i-1}) - gamma_{i-1})
@@ -148,7 +170,7 @@ __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int ch
     {
         C G[K][CH];
 #pragma unroll
-        for (int k = 0; k < K; k++) load_chunk<S, C, CH>(src + k * ld, t0, T, ld, G[k]);
+        for (int k = 0; k < K; k++) load_chunk<S, C, CH, INT>(src + k * ld, t0, T, ld, G[k]);
         C gl[K], gr[K];
         exchange_edges<C, K, CH, NT>(G, lds_x, gl, gr);
 #pragma unroll
@@ -166,7 +188,7 @@ __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int ch
                 for (int k = 0; k < K; k++) {
                     const C vm = c > 0 ? G[k][c - 1] : gl[k];
                     const C vp = c < CH - 1 ? G[k][c + 1] : gr[k];
-                    const C g = alpha * lap2(G[k][c], vm, vp, t, T, interior) + qterm(k, c);
+                    const C g = alpha * lap2(G[k][c], vm, vp, t, T, interior) + Q[k][c];
                     y[k] = G[k][c] - a_apply * g;
                 }
                 project_simplex<K, C>(y);
@@ -175,21 +197,23 @@ __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int ch
             }
         }
     }
-    // level 2: g_i and d_i = P(gamma_i - alpha_i g_i) - gamma_i
-    C Gi[K][CH], Dd[K][CH], off[CH];
+    // level 2: g_i, d_i = P(gamma_i - alpha_i g_i) - gamma_i, and on the owned steps |d|^2, the
+    // off-support term of <g,d>, f_i and the KKT residual of gamma_i
+    C Dd[K][CH];
+    C p_dd = C(0), p_off = C(0), p_f = C(0), p_kkt = C(0);
     {
         C gl[K], gr[K];
         exchange_edges<C, K, CH, NT>(G1, lds_x, gl, gr);
 #pragma unroll
         for (int c = 0; c < CH; c++) {
             const int64_t t = t0 + c;
-            C y[K], z[K];
+            C g[K], y[K], z[K];
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const C vm = c > 0 ? G1[k][c - 1] : gl[k];
                 const C vp = c < CH - 1 ? G1[k][c + 1] : gr[k];
-                Gi[k][c] = alpha * lap2(G1[k][c], vm, vp, t, T, interior) + qterm(k, c);
-                y[k] = G1[k][c] - a_dir * Gi[k][c];
+                g[k] = alpha * lap2(G1[k][c], vm, vp, t, T, interior) + Q[k][c];
+                y[k] = G1[k][c] - a_dir * g[k];
                 z[k] = y[k];
             }
             const C tau = project_simplex<K, C>(z);
@@ -203,45 +227,42 @@ __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int ch
                 Dd[k][c] = z[k] - G1[k][c];
                 if (y[k] <= tau) o += (tau - y[k]) * G1[k][c];
             }
-            off[c] = o;
+            if (owner && (INT || t < T)) {
+                p_off += o;
+#pragma unroll
+                for (int k = 0; k < K; k++) p_dd += Dd[k][c] * Dd[k][c];
+                if (need_f) {
+#pragma unroll
+                    for (int k = 0; k < K; k++) p_f += (g[k] + Q[k][c]) * G1[k][c];
+                }
+                if (want_kkt) {
+#pragma unroll
+                    for (int k = 0; k < K; k++) z[k] = G1[k][c] - s_kkt * g[k];
+                    project_simplex<K, C>(z);
+#pragma unroll
+                    for (int k = 0; k < K; k++) {
+                        const C e = fabs(G1[k][c] - z[k]);
+                        p_kkt = (e > p_kkt || e != e) ? e : p_kkt;
+                    }
+                }
+            }
         }
     }
-    // level 3: reductions on the owned steps, store gamma_i
-    double dad = 0.0, kkt = acc[4];
+    // level 3: d'A d = alpha * sum over edges (t, t+1) of 2 (d_{t+1} - d_t)^2, store gamma_i
+    C p_dad = C(0);
     {
         C dl[K], dr[K];
         exchange_edges<C, K, CH, NT>(Dd, lds_x, dl, dr);
         if (owner) {
-            const double inv_a = 1.0 / (double)a_dir;
 #pragma unroll
             for (int c = 0; c < CH; c++) {
                 const int64_t t = t0 + c;
-                if (t < T) {
-                    C z[K];
-                    double ddt = 0.0;
+                if (INT || interior || t < T - 1) {
 #pragma unroll
                     for (int k = 0; k < K; k++) {
-                        const C d = Dd[k][c];
-                        const C g = Gi[k][c];
-                        ddt += (double)d * (double)d;
-                        if (interior || t < T - 1) {
-                            // d'A d = alpha * sum over edges (t, t+1) of 2 (d_{t+1} - d_t)^2
-                            const C dn = c < CH - 1 ? Dd[k][c + 1] : dr[k];
-                            const C e = dn - d;
-                            dad += (double)(e * e);
-                        }
-                        acc[3] += 0.5 * ((double)g + (double)qterm(k, c)) * (double)G1[k][c];
-                        z[k] = G1[k][c] - s_kkt * g;
-                    }
-                    acc[0] += ddt;
-                    acc[2] -= inv_a * (ddt + (double)off[c]);
-                    if (want_kkt) {
-                        project_simplex<K, C>(z);
-#pragma unroll
-                        for (int k = 0; k < K; k++) {
-                            const double e = fabs((double)G1[k][c] - (double)z[k]);
-                            kkt = (e > kkt || e != e) ? e : kkt;
-                        }
+                        const C dn = c < CH - 1 ? Dd[k][c + 1] : dr[k];
+                        const C e = dn - Dd[k][c];
+                        p_dad += e * e;
                     }
                 }
             }
@@ -252,7 +273,7 @@ __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int ch
 #pragma unroll
                 for (int q = 0; q < CH / VW; q++) {
                     const int64_t t = t0 + q * VW;
-                    if (t + VW <= T) {
+                    if (INT || t + VW <= T) {
                         typename VT::type vec;
 #pragma unroll
                         for (int i = 0; i < VW; i++) VT::set(vec, i, S(G1[k][q * VW + i]));
@@ -266,8 +287,17 @@ __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int ch
             }
         }
     }
-    acc[1] += 2.0 * (double)alpha * dad;
-    acc[4] = want_kkt ? kkt : -1.0;  // -1: not evaluated on this pass
+    const double dd = (double)p_dd;
+    acc[0] += dd;
+    acc[1] += 2.0 * (double)alpha * (double)p_dad;
+    acc[2] -= (dd + (double)p_off) / (double)a_dir;
+    acc[3] += 0.5 * (double)p_f;
+    if (want_kkt) {
+        const double e = (double)p_kkt;
+        acc[4] = (e > acc[4] || e != e) ? e : acc[4];
+    } else {
+        acc[4] = -1.0;  // not evaluated on this pass
+    }
 }
 
 template <typename S, typename C, int K, int CH, int NT>
@@ -285,7 +315,12 @@ __global__ __launch_bounds__(NT) void spg_pass_1d(PassArgs A) {
         for (int m = 0; m < D.msub; m++) {
             const int tile = slot * D.msub + m;
             if ((int64_t)tile * TILE >= D.T) break;
-            pass_tile_1d<S, C, K, CH, NT>(A, inst, tile, lds_x, acc);
+            const int64_t lo = (int64_t)tile * TILE - (int64_t)halo_threads<CH>() * CH;
+            const int64_t hi = (int64_t)(tile + 1) * TILE + (int64_t)halo_threads<CH>() * CH;
+            if (lo >= 1 && hi <= D.T - 1)
+                pass_tile_1d<S, C, K, CH, NT, true>(A, inst, tile, lds_x, acc);
+            else
+                pass_tile_1d<S, C, K, CH, NT, false>(A, inst, tile, lds_x, acc);
         }
         block_reduce_part<NT>(acc, lds_red);
         if (threadIdx.x == 0) {
@@ -334,6 +369,12 @@ __device__ __forceinline__ void grid_gamma_at(const GridArgs &A, const InstDesc 
 template <typename C, int K>
 __device__ __forceinline__ void grid_dataterm(const InstDesc &D, const void *xv, int64_t r,
                                               bool is_f32, C (&q)[K]) {
+    if (D.b64) {  // explicit linear term: q = -b
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            q[k] = is_f32 ? -C(D.b32[k * D.ld + r]) : -C(D.b64[k * D.ld + r]);
+        return;
+    }
     const bool obs = !(D.heldout && D.heldout[r]);
 #pragma unroll
     for (int k = 0; k < K; k++) q[k] = C(0);

@@ -143,7 +143,10 @@ __global__ __launch_bounds__(256) void eval_kernel(const S *gam, const double *x
                 for (int k = 0; k < K; k++) lap[k] += g0[k] - (double)gam[k * D.ld + nb[e]];
             }
         }
-        if (!(D.heldout && D.heldout[t])) {
+        if (D.b64) {
+#pragma unroll
+            for (int k = 0; k < K; k++) q[k] = -D.b64[k * D.ld + t];
+        } else if (!(D.heldout && D.heldout[t])) {
             for (int c = 0; c < D.xdim; c++) {
                 const double xc = x[D.xoff + c * D.ld + t];
 #pragma unroll
@@ -152,11 +155,12 @@ __global__ __launch_bounds__(256) void eval_kernel(const S *gam, const double *x
                     q[k] += dx * dx;
                 }
             }
+#pragma unroll
+            for (int k = 0; k < K; k++) q[k] *= D.bscale;
         }
         double z[K];
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            q[k] *= D.bscale;
             const double g = D.alpha * lap[k] + q[k];
             f += 0.5 * D.alpha * lap[k] * g0[k] + q[k] * g0[k];
             nb2 += q[k] * q[k];

@@ -9,6 +9,7 @@
 #include <pasc/pasc.h>
 
 #include <algorithm>
+#include <iomanip>
 
 using namespace pascinference;
 
@@ -119,7 +120,8 @@ int main(int argc, char **argv) {
         mysolver.printtimer(std::cout);
         std::cout << "--- FINAL SOLVER INFO ---" << std::endl;
         mysolver.printstatus(std::cout);
-        std::cout << "best_epssqr=" << epssqr_best << " abserr=" << abserr_best << std::endl;
+        std::cout << std::setprecision(17) << "best_epssqr=" << epssqr_best
+                  << " abserr=" << abserr_best << std::endl;
     }
     pasc_ctx_destroy(ctx);
     return rc;

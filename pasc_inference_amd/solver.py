@@ -111,6 +111,15 @@ class Problem:
         check(lib().pasc_gamma_solve(self.h, C.byref(p), C.byref(r)))
         return r
 
+    def set_linear_term(self, b, layout: int = capi.LAYOUT_TMAJOR):
+        """Explicit b of the QP (None returns to the data term built from x and theta)."""
+        if b is None:
+            check(lib().pasc_set_linear_term(self.h, None, layout))
+            return
+        self._b = np.ascontiguousarray(b, dtype=np.float64).ravel()
+        assert self._b.size == self.nodes * self.K
+        check(lib().pasc_set_linear_term(self.h, dptr(self._b), layout))
+
     def theta_update(self):
         check(lib().pasc_theta_update(self.h))
         return self.get_theta()

@@ -24,7 +24,7 @@ def test_cpp_driver_epsilon_scan_on_reference_fixture(oracle, tmp_path):
     cmd = [DRIVER, f"--test_filename={x}", f"--test_filename_solution={sol}", "--test_K=2",
            "--test_Theta=1", "--test_Theta=2", f"--test_filename_out={out}",
            f"--test_shortinfo_filename={tmp_path / 'short.csv'}",
-           "--spgqpsolver_stop_difff=0", "--spgqpsolver_stop_kkt=1", "--spgqpsolver_kkt_tol=1e-9",
+           "--spgqpsolver_stop_difff=0", "--spgqpsolver_stop_kkt=1", "--spgqpsolver_kkt_tol=1e-11",
            "--spgqpsolver_maxit=100000"]
     for e in [0.1, 1.0, 10.0, 100.0]:
         cmd.append(f"--test_epssqr={e}")
@@ -44,7 +44,7 @@ def test_cpp_driver_epsilon_scan_on_reference_fixture(oracle, tmp_path):
     g = np.full(T * K, 0.5)
     for e in [0.1, 1.0, 10.0, 100.0]:
         g, _ = oracle.spg_solve(g, b, T, 1, K, e / T, maxit=100000, stop_difff=0, stop_kkt=1,
-                                kkt_tol=1e-10, kkt_scale=T, gd_centered=1, nthreads=8)
+                                kkt_tol=1e-12, kkt_scale=T, gd_centered=1, nthreads=8)
         errs[e] = (oracle.abserr_reconstructed(g, th, truth, T, 1, K), g.copy())
     e_or = min(errs, key=lambda k: errs[k][0])
     assert best_eps == e_or

@@ -76,7 +76,7 @@ def run_c2(args, ctx):
         P32.set_gamma_uniform()
         tune["fp32"] = P32.autotune(args.tune_cache)
         log(args, f"autotune fp32: block={tune['fp32'][0]} chunk={tune['fp32'][1]} "
-                  f"{tune['fp32'][2]:.1f} us/pass")
+                  f"prefetch={tune['fp32'][2]} slots={tune['fp32'][4]} {tune['fp32'][3]:.1f} us/pass")
     params = dict(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt, check_every=32,
                   use_graph=0 if args.no_graph else 1, kkt_every=4)
     for _ in range(args.warmup):
@@ -115,7 +115,7 @@ def run_c2(args, ctx):
         P64.set_gamma_uniform()
         tune["fp64"] = P64.autotune(args.tune_cache)
         log(args, f"autotune fp64: block={tune['fp64'][0]} chunk={tune['fp64'][1]} "
-                  f"{tune['fp64'][2]:.1f} us/pass")
+                  f"prefetch={tune['fp64'][2]} slots={tune['fp64'][4]} {tune['fp64'][3]:.1f} us/pass")
     P64.set_gamma_uniform()
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -325,7 +325,8 @@ def main():
         "plain_fp64_time_to_kkt_s": c2["t64"],
         "fp64_graph_time_to_kkt_s": c2["t64g"],
         "fp64_graph_iterations": c2["r64g"].it,
-        "autotune": {k: {"block": v[0], "chunk": v[1], "us_per_pass": v[2]}
+        "autotune": {k: {"block": v[0], "chunk": v[1], "prefetch": v[2], "us_per_pass": v[3],
+                         "max_slots": v[4]}
                      for k, v in c2["tune"].items()},
         "speedup_vs_plain_fp64": c2["t64"] / best,
         # whole solve against the roofline, 6*n*s algorithmic bytes per SPG iteration

@@ -257,14 +257,25 @@ int pasc_set_profiling(pasc_problem *p, int enable);
  * achievable HBM rate of this device, and the calibration kernel for traffic counters ---- */
 int pasc_device_copy_bandwidth(pasc_ctx *ctx, int64_t bytes, int32_t reps, double *gbps);
 
+/* ---- launch shape of the fused 1-D pass ---- */
+typedef struct {
+    int32_t block;       /* threads per workgroup: 128, 256 or 512 */
+    int32_t chunk;       /* time steps per thread: FP32-bulk 4 or 8, FP64 2 or 4 */
+    int32_t prefetch;    /* 1: a workgroup loads its next tile while it computes the current one */
+    int32_t max_slots;   /* partial-sum slots per instance, 0 = 2048; each slot is one
+                            workgroup's run of consecutive tiles */
+    double us_per_pass;  /* pasc_autotune: measured average pass time (0 when taken from a cache) */
+} pasc_launch_shape;
+/* Pin the launch shape of later solves (the iterate is kept). */
+int pasc_set_launch_shape(pasc_problem *p, const pasc_launch_shape *shape);
+
 /* ---- autotuner ----
- * Times the fused SPG pass for the problem's (K, precision, T) over the candidate
- * (threads per block, time steps per thread) pairs and keeps the fastest for later solves.
+ * Times the fused SPG pass for the problem's (K, precision, T) over the candidate launch
+ * shapes (block x chunk x prefetch) and keeps the fastest for later solves.
  * The choice is cached per context.  With a non-NULL cache_path, a choice recorded there for
  * the same key (device, K, precision, log2 T) by an earlier process is reused without timing,
  * and every call appends its choice to the file (one JSON object per line). */
-int pasc_autotune(pasc_problem *p, const char *cache_path, int32_t *block, int32_t *chunk,
-                  double *us_per_pass);
+int pasc_autotune(pasc_problem *p, const char *cache_path, pasc_launch_shape *shape);
 
 /* ---- synthetic data (deterministic SplitMix64 + Box-Muller generator) ---- */
 int pasc_generate_markov(uint64_t seed, int64_t T, int32_t K, const double *levels,

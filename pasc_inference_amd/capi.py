@@ -76,6 +76,11 @@ class SeriesDesc(C.Structure):
                 ("theta", C.POINTER(C.c_double))]
 
 
+class LaunchShape(C.Structure):
+    _fields_ = [("block", C.c_int32), ("chunk", C.c_int32), ("prefetch", C.c_int32),
+                ("max_slots", C.c_int32), ("us_per_pass", C.c_double)]
+
+
 class InstanceResult(C.Structure):
     _fields_ = [("it", C.c_int32), ("it_fp32", C.c_int32), ("it_fp64", C.c_int32),
                 ("stop_reason", C.c_int32), ("status", C.c_int32), ("fx", C.c_double),
@@ -124,8 +129,8 @@ _SIGS = {
                      C.c_int),
     "pasc_set_profiling": ([vp, C.c_int], C.c_int),
     "pasc_device_copy_bandwidth": ([vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)], C.c_int),
-    "pasc_autotune": ([vp, C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
-                       C.POINTER(C.c_double)], C.c_int),
+    "pasc_autotune": ([vp, C.c_char_p, C.POINTER(LaunchShape)], C.c_int),
+    "pasc_set_launch_shape": ([vp, C.POINTER(LaunchShape)], C.c_int),
     "pasc_generate_markov": ([C.c_uint64, C.c_int64, C.c_int32, C.POINTER(C.c_double),
                               C.c_double, C.c_double, C.POINTER(C.c_double),
                               C.POINTER(C.c_int32)], C.c_int),

@@ -597,16 +597,23 @@ int pasc_set_profiling(pasc_problem *p, int enable) {
     return PASC_OK;
 }
 
-int pasc_autotune(pasc_problem *p, const char *cache_path, int32_t *block, int32_t *chunk,
-                  double *us_per_pass) {
+int pasc_set_launch_shape(pasc_problem *p, const pasc_launch_shape *shape) {
+    clear_error();
+    if (!p || !shape) return fail(PASC_ERR_ARG, "set_launch_shape: NULL argument");
+    if (p->grid_width > 0) return fail(PASC_ERR_ARG, "set_launch_shape: the grid path has a fixed shape");
+    int rc = use_device(p->ctx);
+    if (rc) return rc;
+    return engine_set_shape(p->e, shape->block, shape->chunk, shape->prefetch, shape->max_slots);
+}
+
+int pasc_autotune(pasc_problem *p, const char *cache_path, pasc_launch_shape *shape) {
     clear_error();
     if (!p) return fail(PASC_ERR_ARG, "autotune: NULL problem");
     if (p->grid_width > 0) return fail(PASC_ERR_ARG, "autotune: the grid path has a fixed shape");
-    if (!p->theta_set) return fail(PASC_ERR_STATE, "autotune: theta was never set");
+    if (!p->theta_set && !p->b_set) return fail(PASC_ERR_STATE, "autotune: neither theta nor b was set");
     int rc = use_device(p->ctx);
     if (rc) return rc;
-    int bl = 0, ch = 0;
-    double us = 0;
+    pasc_launch_shape sh{0, 0, 0, 0, 0.0};
     // the choice is cached per context, keyed by device, K, precision and log2(T)
     char key[128];
     snprintf(key, sizeof(key), "%s/K%d/p%d/T2^%d", p->ctx->c.arch.c_str(), p->K, p->precision,
@@ -614,8 +621,8 @@ int pasc_autotune(pasc_problem *p, const char *cache_path, int32_t *block, int32
     bool hit = false;
     for (auto &kv : p->ctx->c.tune)
         if (kv.first == key) {
-            bl = kv.second.first;
-            ch = kv.second.second;
+            sh = kv.second;
+            sh.us_per_pass = 0.0;
             hit = true;
         }
     if (!hit && cache_path) {
@@ -628,32 +635,38 @@ int pasc_autotune(pasc_problem *p, const char *cache_path, int32_t *block, int32
                 if (!strstr(line, want)) continue;
                 const char *pb = strstr(line, "\"block\": ");
                 const char *pc = strstr(line, "\"chunk\": ");
-                int b2 = 0, c2 = 0;
+                const char *pp = strstr(line, "\"prefetch\": ");
+                const char *ps = strstr(line, "\"max_slots\": ");
+                int b2 = 0, c2 = 0, p2 = 0, s2 = 0;
                 if (pb && pc && sscanf(pb + 9, "%d", &b2) == 1 && sscanf(pc + 9, "%d", &c2) == 1) {
-                    bl = b2;
-                    ch = c2;
+                    if (!(pp && sscanf(pp + 12, "%d", &p2) == 1)) p2 = 0;
+                    if (!(ps && sscanf(ps + 13, "%d", &s2) == 1)) s2 = 0;
+                    sh = pasc_launch_shape{b2, c2, p2, s2, 0.0};
                     hit = true;
                 }
             }
             fclose(f);
-            if (hit) p->ctx->c.tune.push_back({key, {bl, ch}});
+            if (hit) p->ctx->c.tune.push_back({key, sh});
         }
     }
     if (hit) {
-        rc = engine_set_shape(p->e, bl, ch);
+        rc = engine_set_shape(p->e, sh.block, sh.chunk, sh.prefetch, sh.max_slots);
     } else {
-        rc = engine_autotune(p->e, &bl, &ch, &us);
-        if (rc == PASC_OK) p->ctx->c.tune.push_back({key, {bl, ch}});
+        int bl = 0, ch = 0, pf = 0, ms = 0;
+        double us = 0.0;
+        rc = engine_autotune(p->e, &bl, &ch, &pf, &ms, &us);
+        sh = pasc_launch_shape{bl, ch, pf, ms, us};
+        if (rc == PASC_OK) p->ctx->c.tune.push_back({key, sh});
     }
     if (rc) return rc;
-    if (block) *block = bl;
-    if (chunk) *chunk = ch;
-    if (us_per_pass) *us_per_pass = us;
+    if (shape) *shape = sh;
     if (cache_path) {
         FILE *f = fopen(cache_path, "a");
         if (f) {
-            fprintf(f, "{\"key\": \"%s\", \"T\": %lld, \"block\": %d, \"chunk\": %d, \"us_per_pass\": %.3f, \"cached\": %d}\n",
-                    key, (long long)p->T, bl, ch, us, hit ? 1 : 0);
+            fprintf(f, "{\"key\": \"%s\", \"T\": %lld, \"block\": %d, \"chunk\": %d, \"prefetch\": %d, "
+                       "\"max_slots\": %d, \"us_per_pass\": %.3f, \"cached\": %d}\n",
+                    key, (long long)p->T, sh.block, sh.chunk, sh.prefetch, sh.max_slots,
+                    sh.us_per_pass, hit ? 1 : 0);
             fclose(f);
         }
     }

@@ -9,6 +9,14 @@
 
 namespace pasc {
 
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS operations and then for the
+// other waves, but not for its outstanding global loads and stores (the fence that
+// __syncthreads() adds would drain those too, including an asynchronous global -> LDS copy of
+// the next tile that is meant to stay in flight across the barrier).
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 constexpr int WAVE = 64;
 
 template <typename S> struct V16;
@@ -185,6 +193,26 @@ __device__ inline void find_tile(const GroupView &g, int j, int &inst, int &chun
     while (g.off[a + 1] <= j) a++;
     inst = g.ids[a];
     chunk = j - g.off[a];
+}
+
+// Wave-uniform copies in scalar registers (the value of the first active lane).  Values that
+// every lane already holds equal become scalars, so later uses wait on nothing.
+__device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uni(int64_t v) {
+    const uint64_t u = (uint64_t)v;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)u);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)(u >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double uni(double v) {
+    return __builtin_bit_cast(double, uni(__builtin_bit_cast(int64_t, v)));
+}
+__device__ __forceinline__ float uni(float v) {
+    return __builtin_bit_cast(float, uni(__builtin_bit_cast(int32_t, v)));
+}
+template <typename P>
+__device__ __forceinline__ P *uni(P *v) {
+    return reinterpret_cast<P *>(uni((int64_t)reinterpret_cast<uintptr_t>(v)));
 }
 
 }  // namespace pasc

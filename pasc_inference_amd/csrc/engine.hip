@@ -33,25 +33,25 @@ static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 // kernel dispatch tables
 // ------------------------------------------------------------------------------------------
 // instantiated in pass_inst.hip / grid_inst.hip (one object per storage type and block size)
-int launch_pass_1d_float_128_(int K, const PassArgs &a, int grid, int chunk, hipStream_t s);
-int launch_pass_1d_float_256_(int K, const PassArgs &a, int grid, int chunk, hipStream_t s);
-int launch_pass_1d_float_512_(int K, const PassArgs &a, int grid, int chunk, hipStream_t s);
-int launch_pass_1d_double_128_(int K, const PassArgs &a, int grid, int chunk, hipStream_t s);
-int launch_pass_1d_double_256_(int K, const PassArgs &a, int grid, int chunk, hipStream_t s);
-int launch_pass_1d_double_512_(int K, const PassArgs &a, int grid, int chunk, hipStream_t s);
+int launch_pass_1d_float_128_(int K, const PassArgs &a, int grid, int chunk, int pf, hipStream_t s);
+int launch_pass_1d_float_256_(int K, const PassArgs &a, int grid, int chunk, int pf, hipStream_t s);
+int launch_pass_1d_float_512_(int K, const PassArgs &a, int grid, int chunk, int pf, hipStream_t s);
+int launch_pass_1d_double_128_(int K, const PassArgs &a, int grid, int chunk, int pf, hipStream_t s);
+int launch_pass_1d_double_256_(int K, const PassArgs &a, int grid, int chunk, int pf, hipStream_t s);
+int launch_pass_1d_double_512_(int K, const PassArgs &a, int grid, int chunk, int pf, hipStream_t s);
 int launch_pass_grid_float(int K, const GridArgs &a, int grid, hipStream_t s);
 int launch_pass_grid_double(int K, const GridArgs &a, int grid, hipStream_t s);
 
 static int launch_pass_1d(bool f64, int K, const PassArgs &a, int grid, int block, int chunk,
-                          hipStream_t s) {
+                          int pf, hipStream_t s) {
     if (f64) {
-        if (block == 128) return launch_pass_1d_double_128_(K, a, grid, chunk, s);
-        if (block == 256) return launch_pass_1d_double_256_(K, a, grid, chunk, s);
-        if (block == 512) return launch_pass_1d_double_512_(K, a, grid, chunk, s);
+        if (block == 128) return launch_pass_1d_double_128_(K, a, grid, chunk, pf, s);
+        if (block == 256) return launch_pass_1d_double_256_(K, a, grid, chunk, pf, s);
+        if (block == 512) return launch_pass_1d_double_512_(K, a, grid, chunk, pf, s);
     } else {
-        if (block == 128) return launch_pass_1d_float_128_(K, a, grid, chunk, s);
-        if (block == 256) return launch_pass_1d_float_256_(K, a, grid, chunk, s);
-        if (block == 512) return launch_pass_1d_float_512_(K, a, grid, chunk, s);
+        if (block == 128) return launch_pass_1d_float_128_(K, a, grid, chunk, pf, s);
+        if (block == 256) return launch_pass_1d_float_256_(K, a, grid, chunk, pf, s);
+        if (block == 512) return launch_pass_1d_float_512_(K, a, grid, chunk, pf, s);
     }
     return fail(PASC_ERR_ARG, "no fused pass for block %d", block);
 }
@@ -72,10 +72,11 @@ static int engine_retile(Engine &e, int tl) {
     e.cur_tile = tl;
     int slots = 0;
     for (auto &d : e.h_desc) {
-        // at most ~2048 partial slots per instance: enough workgroups to fill the device, few
-        // enough that the step kernel reduces them quickly
+        // at most max_slots (default 2048) partial slots per instance: enough workgroups to fill
+        // the device, few enough that the step kernel reduces them quickly
         const int64_t raw = (d.T + tl - 1) / tl;
-        d.msub = (int32_t)std::max<int64_t>(1, (raw + 2047) / 2048);
+        const int64_t ms = std::max(e.max_slots, 1);
+        d.msub = (int32_t)std::max<int64_t>(1, (raw + ms - 1) / ms);
         d.tile0 = slots;
         d.ntiles = (int32_t)((raw + d.msub - 1) / d.msub);
         slots += d.ntiles;
@@ -365,9 +366,11 @@ static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<in
         const GroupView gv{g.d_ids, g.d_off, g.d_hint, g.n};
         int rc;
         if (e.stencil == STENCIL_TIME) {
+            const void *bb = ph.f64 ? e.d_b64 : e.d_b32;
             PassArgs a{g0, g1, x, e.d_desc, e.d_state, e.d_part, gv, ph.kkt_every,
-                       ph.sp.fx_identity ? 0 : 1};
-            rc = launch_pass_1d(ph.f64, g.K, a, grids[gi], e.block, ph.f64 ? e.chunk64() : e.chunk, s);
+                       ph.sp.fx_identity ? 0 : 1, bb};
+            rc = launch_pass_1d(ph.f64, g.K, a, grids[gi], e.block, ph.f64 ? e.chunk64() : e.chunk,
+                                e.prefetch, s);
         } else {
             void *d0 = ph.f64 && e.precision == PASC_FP32_BULK ? e.d_dir64[0] : e.d_dir[0];
             void *d1 = ph.f64 && e.precision == PASC_FP32_BULK ? e.d_dir64[1] : e.d_dir[1];
@@ -852,11 +855,15 @@ int engine_fill_uniform(Engine &e) {
     return PASC_OK;
 }
 
-int engine_set_shape(Engine &e, int block, int chunk) {
+int engine_set_shape(Engine &e, int block, int chunk, int prefetch, int max_slots) {
     const bool f32 = e.precision == PASC_FP32_BULK;
     const bool ok_chunk = f32 ? (chunk == 4 || chunk == 8) : (chunk == 2 || chunk == 4);
-    if (!(block == 128 || block == 256 || block == 512) || !ok_chunk)
-        return fail(PASC_ERR_ARG, "no fused pass for block %d chunk %d", block, chunk);
+    if (!(block == 128 || block == 256 || block == 512) || !ok_chunk || (prefetch != 0 && prefetch != 1))
+        return fail(PASC_ERR_ARG, "no fused pass for block %d chunk %d prefetch %d", block, chunk,
+                    prefetch);
+    if (max_slots < 0) return fail(PASC_ERR_ARG, "max_slots < 0");
+    e.prefetch = prefetch;
+    e.max_slots = max_slots > 0 ? max_slots : 2048;
     std::vector<std::vector<double>> saved(e.n_inst);
     for (int i = 0; i < e.n_inst; i++) {
         int rc = download_rows(e, i, saved[i]);
@@ -870,9 +877,11 @@ int engine_set_shape(Engine &e, int block, int chunk) {
 }
 
 // Self-written autotuner: times the fused pass of this problem for every compiled
-// (threads per block, time steps per thread) pair -- 20 passes each from the current iterate,
-// hipEvents around every launch -- and keeps the fastest.  The iterate is restored afterwards.
-int engine_autotune(Engine &e, int *block, int *chunk, double *us) {
+// (threads per block, time steps per thread, prefetch) triple, and with prefetch also for
+// 2048 / 1024 / 512 partial slots per instance (more tiles per workgroup lengthen its
+// pipeline) -- 20 passes each from the current iterate with the KKT residual on every 4th
+// pass, hipEvents around every launch -- and keeps the fastest.  The iterate is restored.
+int engine_autotune(Engine &e, int *block, int *chunk, int *prefetch, int *max_slots, double *us) {
     if (e.stencil != STENCIL_TIME) return fail(PASC_ERR_ARG, "autotune: 1-D time graph only");
     std::vector<std::vector<double>> saved(e.n_inst);
     for (int i = 0; i < e.n_inst; i++) {
@@ -887,14 +896,20 @@ int engine_autotune(Engine &e, int *block, int *chunk, double *us) {
     p.maxit = 20;
     p.stop_mask = 0;
     p.polish_maxit = 1;
+    p.kkt_every = 4;
     const bool prof = e.profile;
     e.profile = true;
     double best = 1e300;
-    int bb = e.block, bc = e.chunk;
+    int bb = e.block, bc = e.chunk, bp = e.prefetch, bs = e.max_slots;
+    const int slot_opts[3] = {2048, 1024, 512};
+    for (int pi = 0; pi < 2; pi++)
+    for (int si = 0; si < (pi ? 3 : 1); si++)
     for (int ci = 0; ci < 2; ci++) {
         for (int bi = 0; bi < 3; bi++) {
             e.block = blocks[bi];
             e.chunk = chunks[ci];
+            e.prefetch = pi;
+            e.max_slots = slot_opts[si];
             int rc = engine_retile(e, tile_len(e, !f32));
             for (int i = 0; i < e.n_inst && rc == PASC_OK; i++) rc = upload_rows(e, i, saved[i]);
             pasc_spg_result r;
@@ -909,17 +924,23 @@ int engine_autotune(Engine &e, int *block, int *chunk, double *us) {
                 best = t;
                 bb = e.block;
                 bc = e.chunk;
+                bp = e.prefetch;
+                bs = e.max_slots;
             }
         }
     }
     e.profile = prof;
     e.block = bb;
     e.chunk = bc;
+    e.prefetch = bp;
+    e.max_slots = bs;
     int rc = engine_retile(e, tile_len(e, !f32));
     for (int i = 0; i < e.n_inst && rc == PASC_OK; i++) rc = upload_rows(e, i, saved[i]);
     if (rc) return rc;
     *block = bb;
     *chunk = bc;
+    *prefetch = bp;
+    *max_slots = bs;
     *us = best * 1e3;
     return PASC_OK;
 }

@@ -118,6 +118,8 @@ struct Engine {
     int32_t *h_nactive = nullptr;        // pinned mirror
     int block = 256;                     // threads per block of the fused 1-D pass
     int chunk = 8;                       // time steps per thread (of the storage type)
+    int prefetch = 0;                    // 1: pass kernel loads the next tile ahead
+    int max_slots = 2048;                // partial slots per instance (tiles per slot follow)
     int grid_cap = 0;                    // persistent grid cap (0: 32 blocks per CU)
     int cur_tile = 0;                    // time steps per tile the partial slots are sized for
     bool profile = false;                // hipEvents around every pass launch
@@ -132,8 +134,8 @@ struct Ctx {
     bool own_stream = false;
     int n_cu = 0;
     std::string arch;
-    // autotune cache: key -> (block, chunk)
-    std::vector<std::pair<std::string, std::pair<int, int>>> tune;
+    // autotune cache: key -> launch shape
+    std::vector<std::pair<std::string, pasc_launch_shape>> tune;
 };
 
 // engine.hip
@@ -156,8 +158,8 @@ int engine_labels(Engine &e, int inst, int32_t *labels);
 int engine_abserr(Engine &e, int inst, const double *solution, double *abserr);
 int engine_adjoint(Engine &e, int inst, const uint8_t *mask, double cg_tol, int cg_maxit,
                    double *loss, double *dlogeps, double *dtheta, int *cg_it);
-int engine_autotune(Engine &e, int *block, int *chunk, double *us);
-int engine_set_shape(Engine &e, int block, int chunk);
+int engine_autotune(Engine &e, int *block, int *chunk, int *prefetch, int *max_slots, double *us);
+int engine_set_shape(Engine &e, int block, int chunk, int prefetch, int max_slots);
 
 int device_copy_bandwidth(Ctx *ctx, int64_t bytes, int reps, double *gbps);
 int kernel_project(Ctx *ctx, void *x, int64_t nblocks, int K, int dtype, int layout, int64_t ld);

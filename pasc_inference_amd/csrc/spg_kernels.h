@@ -30,6 +30,7 @@ struct PassArgs {
     GroupView grp;
     int32_t kkt_every;  // evaluate the KKT residual on passes with it % kkt_every == 0
     int32_t need_f;     // 0: f comes from the quadratic identity (only the first pass sums it)
+    const void *b;      // explicit linear terms of the phase's type, indexed like gamma (or null)
 };
 
 template <int CH>
@@ -75,13 +76,13 @@ __device__ __forceinline__ void exchange_edges(const C (&v)[K][CH], C *lds, C (&
         lds[(2 * k) * NT + tid] = v[k][0];
         lds[(2 * k + 1) * NT + tid] = v[k][CH - 1];
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int k = 0; k < K; k++) {
         left[k] = tid > 0 ? lds[(2 * k + 1) * NT + tid - 1] : C(0);
         right[k] = tid < NT - 1 ? lds[(2 * k) * NT + tid + 1] : C(0);
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 template <typename C>
@@ -98,62 +99,169 @@ __device__ __forceinline__ C lap2(C v, C vm, C vp, int64_t t, int64_t T, bool in
 // One tile of the fused pass.  INT: every step of the tile, halos included, lies in [1, T-2]
 // (the common case; no bounds or boundary-row tests).  Per-thread partials accumulate in the
 // compute type C over the thread's owned steps and enter the FP64 slot sums once per tile.
-template <typename S, typename C, int K, int CH, int NT, bool INT>
-__device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int chunk,
-                                             C *lds_x, double (&acc)[NPART]) {
+template <typename S, typename C, int K, int CH, int NT>
+__device__ __forceinline__ bool tile_is_interior(int64_t tile, int64_t T) {
+    constexpr int64_t TILE = tile_steps<NT, CH>();
+    constexpr int64_t H = (int64_t)halo_threads<CH>() * CH;
+    return tile * TILE - H >= 1 && (tile + 1) * TILE + H <= T - 1;
+}
+
+// Per-instance constants of one pass, read once per partial slot into scalar registers (the
+// descriptor and state loads are then never waited for inside a tile, where such a wait would
+// also drain the asynchronous copies of the next tile).
+template <typename S, typename C, int K>
+struct TileCtx {
+    int64_t T, ld;
+    const S *src;          // gamma_{i-1} rows
+    S *dst;                // gamma_i rows
+    const S *xr;           // data row
+    const S *br;           // explicit b rows or nullptr
+    const uint8_t *heldout;
+    C alpha, bscale, s_kkt, a_apply, b_apply, a_dir;
+    C theta[K];
+    bool first, need_f, want_kkt;
+};
+
+template <typename S, typename C, int K>
+__device__ __forceinline__ void make_tile_ctx(const PassArgs &A, const InstDesc &D,
+                                              const InstState &st, TileCtx<S, C, K> &c) {
+    // pointers are formed from the kernel arguments plus scalar offsets, so that they stay
+    // global-memory pointers in scalar registers
+    const int par = uni(st.parity);
+    const int64_t goff = uni(D.goff);
+    c.T = uni(D.T);
+    c.ld = uni(D.ld);
+    c.src = reinterpret_cast<const S *>(par ? A.gamma1 : A.gamma0) + goff;
+    c.dst = reinterpret_cast<S *>(par ? A.gamma0 : A.gamma1) + goff;
+    c.xr = reinterpret_cast<const S *>(A.x) + uni(D.xoff);
+    const int has_b = uni(D.b64 != nullptr ? 1 : 0);
+    c.br = has_b ? reinterpret_cast<const S *>(A.b) + goff : nullptr;
+    c.heldout = D.heldout;
+    c.alpha = uni(C(D.alpha));
+    c.bscale = uni(C(D.bscale));
+    c.s_kkt = uni(C(D.s_kkt));
+    c.a_apply = uni(C(st.alpha_apply));
+    c.b_apply = uni(C(st.beta_apply));
+    c.a_dir = uni(C(st.alpha_dir));
+#pragma unroll
+    for (int k = 0; k < K; k++) c.theta[k] = uni(C(D.theta[k]));
+    c.first = uni(st.first) != 0;
+    c.need_f = A.need_f || c.first;
+    const int it = uni(st.it);
+    c.want_kkt = (it % (A.kkt_every > 0 ? A.kkt_every : 1)) == 0;
+}
+
+// ---- staging of a tile in LDS (asynchronous global -> LDS copies) ----------------------------
+// The stage holds the K gamma rows and the data row of one tile, each NT*CH contiguous elements
+// (steps T0 - HT*CH .. T0 + TILE + HT*CH), in the order they lie in HBM.  global_load_lds_dwordx4
+// writes the 16 bytes each lane loads at (LDS base + 16 * lane), so wave w's q-th copy of a row
+// covers the row's 16-byte pieces (w*Q + q)*64 .. +63, Q = CH*sizeof(S)/16 per wave and row.
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+template <typename S, int K, int CH, int NT>
+struct Stage {
+    static constexpr int ROW = NT * CH;             // elements per staged row
+    static constexpr int Q = CH * (int)sizeof(S) / 16;  // 16-byte copies per lane and row
+    static constexpr int ROWS = K + 1;              // K gamma rows + the data row
+    static constexpr size_t BYTES = (size_t)ROWS * ROW * sizeof(S);
+    static_assert(Q >= 1 && CH * sizeof(S) % 16 == 0, "a thread's chunk must be whole 16-byte pieces");
+};
+
+// Issues the copies of one interior tile into the stage (every lane of every wave takes part).
+// The data row is copied also when the instance has an explicit b (then it is not read), so
+// the number of copies in flight is the same on every path.
+template <typename S, typename C, int K, int CH, int NT>
+__device__ __forceinline__ void stage_issue(const TileCtx<S, C, K> &c, int tile, S *stage) {
+    using SG = Stage<S, K, CH, NT>;
     constexpr int HT = halo_threads<CH>();
     constexpr int TILE = tile_steps<NT, CH>();
-    const InstDesc &D = A.desc[inst];
-    const InstState &st = A.state[inst];
-    const int64_t T = D.T;
-    const int64_t ld = D.ld;
+    const int64_t lo = (int64_t)tile * TILE - (int64_t)HT * CH;  // >= 1 for an interior tile
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+    for (int r = 0; r < K + 1; r++) {
+        const char *g = reinterpret_cast<const char *>(r < K ? c.src + (int64_t)r * c.ld + lo
+                                                             : c.xr + lo);
+        char *l = reinterpret_cast<char *>(stage + (size_t)r * SG::ROW);
+#pragma unroll
+        for (int q = 0; q < SG::Q; q++) {
+            const int piece = (w * SG::Q + q) * WAVE;
+            __builtin_amdgcn_global_load_lds(
+                reinterpret_cast<const void *>(g + (size_t)(piece + lane) * 16),
+                (lds_void_t *)(l + (size_t)piece * 16), 16, 0, 0);
+        }
+    }
+}
+
+// This thread's CH steps of staged row r.
+template <typename S, typename C, int K, int CH, int NT>
+__device__ __forceinline__ void stage_read(const S *stage, int r, C (&v)[CH]) {
+    using VT = V16<S>;
+    constexpr int VW = VT::n;
+    const S *row = stage + (size_t)r * Stage<S, K, CH, NT>::ROW + threadIdx.x * CH;
+#pragma unroll
+    for (int q = 0; q < CH / VW; q++) {
+        const typename VT::type vec = *reinterpret_cast<const typename VT::type *>(row + q * VW);
+#pragma unroll
+        for (int i = 0; i < VW; i++) v[q * VW + i] = C(VT::get(vec, i));
+    }
+}
+
+// STAGED: the tile's gamma and data rows were copied into the LDS stage (interior tiles
+// only).  next: a callable run once every thread has taken the stage's rows into registers
+// (it issues the copies of the slot's next tile into the stage).
+template <typename S, typename C, int K, int CH, int NT, bool INT, bool STAGED, typename Next>
+__device__ __forceinline__ void pass_tile_1d(const TileCtx<S, C, K> &cx, int chunk, C *lds_x,
+                                             double (&acc)[NPART], const S *stage, Next next) {
+    static_assert(!STAGED || INT, "only interior tiles are staged");
+    constexpr int HT = halo_threads<CH>();
+    constexpr int TILE = tile_steps<NT, CH>();
+    const int64_t T = cx.T;
+    const int64_t ld = cx.ld;
     const int tid = threadIdx.x;
     const int64_t T0 = (int64_t)chunk * TILE;
     const int64_t t0 = T0 - (int64_t)HT * CH + (int64_t)tid * CH;
     const bool owner = tid >= HT && tid < NT - HT;  // owned steps are stored and reduced
-    const int par = st.parity;
-    const S *src = reinterpret_cast<const S *>(par ? A.gamma1 : A.gamma0) + D.goff;
-    S *dst = reinterpret_cast<S *>(par ? A.gamma0 : A.gamma1) + D.goff;
-    const S *xr = reinterpret_cast<const S *>(A.x) + D.xoff;
-
-    const C alpha = C(D.alpha);
-    const C bscale = C(D.bscale);
-    const C s_kkt = C(D.s_kkt);
-    const C a_apply = C(st.alpha_apply);
-    const C b_apply = C(st.beta_apply);
-    const C a_dir = C(st.alpha_dir);
-    const bool first = st.first != 0;
-    const bool need_f = A.need_f || first;
+    const S *src = cx.src;
+    S *dst = cx.dst;
+    const C alpha = cx.alpha;
+    const C bscale = cx.bscale;
+    const C s_kkt = cx.s_kkt;
+    const C a_apply = cx.a_apply;
+    const C b_apply = cx.b_apply;
+    const C a_dir = cx.a_dir;
+    const bool first = cx.first;
+    const bool need_f = cx.need_f;
 
     // data term q = -b of every step of the chunk, formed once for the two levels that use it:
     // bscale*(x - theta_k)^2 (zero when held out or outside [0,T)), or read from an explicit b
     C Q[K][CH];
-    if (D.b64) {
-        const S *br = sizeof(S) == 4 ? reinterpret_cast<const S *>(D.b32)
-                                     : reinterpret_cast<const S *>(D.b64);
+    if (cx.br) {
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            load_chunk<S, C, CH, INT>(br + k * ld, t0, T, ld, Q[k]);
+            load_chunk<S, C, CH, INT>(cx.br + k * ld, t0, T, ld, Q[k]);
 #pragma unroll
             for (int c = 0; c < CH; c++) Q[k][c] = -Q[k][c];
         }
     } else {
         C X[CH];
-        load_chunk<S, C, CH, INT>(xr, t0, T, ld, X);
+        if (STAGED)
+            stage_read<S, C, K, CH, NT>(stage, K, X);
+        else
+            load_chunk<S, C, CH, INT>(cx.xr, t0, T, ld, X);
         bool obs[CH];
 #pragma unroll
         for (int c = 0; c < CH; c++) {
             const int64_t t = t0 + c;
             obs[c] = INT || (t >= 0 && t < T);
         }
-        if (D.heldout) {
+        if (cx.heldout) {
 #pragma unroll
             for (int c = 0; c < CH; c++)
-                if (obs[c]) obs[c] = D.heldout[t0 + c] == 0;
+                if (obs[c]) obs[c] = cx.heldout[t0 + c] == 0;
         }
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            const C th = C(D.theta[k]);
+            const C th = cx.theta[k];
 #pragma unroll
             for (int c = 0; c < CH; c++) {
                 const C dx = X[c] - th;
@@ -163,16 +271,22 @@ __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int ch
     }
     // every step of the chunk is interior (both time neighbours exist): no boundary selects
     const bool interior = INT || (t0 >= 1 && t0 + CH <= T - 1);
-    const bool want_kkt = (st.it % (A.kkt_every > 0 ? A.kkt_every : 1)) == 0;
+    const bool want_kkt = cx.want_kkt;
 
     // level 1: gamma_i = gamma_{i-1} + beta_{i-1} (P(gamma_{i-1} - alpha_{i-1} g_{i-1}) - gamma_{i-1})
     C G1[K][CH];
     {
         C G[K][CH];
 #pragma unroll
-        for (int k = 0; k < K; k++) load_chunk<S, C, CH, INT>(src + k * ld, t0, T, ld, G[k]);
+        for (int k = 0; k < K; k++) {
+            if (STAGED)
+                stage_read<S, C, K, CH, NT>(stage, k, G[k]);
+            else
+                load_chunk<S, C, CH, INT>(src + k * ld, t0, T, ld, G[k]);
+        }
         C gl[K], gr[K];
         exchange_edges<C, K, CH, NT>(G, lds_x, gl, gr);
+        next();  // the exchange's barriers order every stage read before the next copies
 #pragma unroll
         for (int c = 0; c < CH; c++) {
             const int64_t t = t0 + c;
@@ -300,31 +414,64 @@ __device__ __forceinline__ void pass_tile_1d(const PassArgs &A, int inst, int ch
     }
 }
 
-template <typename S, typename C, int K, int CH, int NT>
+// PF: software pipelining across the msub tiles of a slot.  While a tile is computed, the
+// next interior tile's rows are copied into an LDS stage by the asynchronous global -> LDS path
+// (no registers held for it); the wait for those copies is the only added synchronisation.
+// Configurations whose stage would exceed PASC_STAGE_MAX_BYTES run without it.
+constexpr size_t PASC_STAGE_MAX_BYTES = 40 * 1024;
+
+template <typename S, typename C, int K, int CH, int NT, bool PF>
 __global__ __launch_bounds__(NT) void spg_pass_1d(PassArgs A) {
+    using SG = Stage<S, K, CH, NT>;
+    constexpr bool USE_STAGE = PF && SG::BYTES <= PASC_STAGE_MAX_BYTES;
     __shared__ double lds_red[NPART * (NT / WAVE)];
     __shared__ C lds_x[2 * K * NT];
+    __shared__ __attribute__((aligned(16))) S stage[USE_STAGE ? SG::ROWS * SG::ROW : 1];
     constexpr int TILE = tile_steps<NT, CH>();
     const int total = A.grp.off[A.grp.n];
     for (int j = blockIdx.x; j < total; j += gridDim.x) {
         int inst, slot;
         find_tile(A.grp, j, inst, slot);
-        const InstDesc &D = A.desc[inst];
+        inst = uni(inst);
+        slot = uni(slot);
+        TileCtx<S, C, K> cx;
+        make_tile_ctx<S, C, K>(A, A.desc[inst], A.state[inst], cx);
+        const int msub = uni(A.desc[inst].msub);
+        const int32_t slot_out = uni(A.desc[inst].tile0) + slot;
         // one partial slot covers msub consecutive tiles of the instance
         double acc[NPART] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int m = 0; m < D.msub; m++) {
-            const int tile = slot * D.msub + m;
-            if ((int64_t)tile * TILE >= D.T) break;
-            const int64_t lo = (int64_t)tile * TILE - (int64_t)halo_threads<CH>() * CH;
-            const int64_t hi = (int64_t)(tile + 1) * TILE + (int64_t)halo_threads<CH>() * CH;
-            if (lo >= 1 && hi <= D.T - 1)
-                pass_tile_1d<S, C, K, CH, NT, true>(A, inst, tile, lds_x, acc);
+        const int first_tile = slot * msub;
+        bool staged = false;  // the current tile's rows are in the stage
+        if (USE_STAGE && tile_is_interior<S, C, K, CH, NT>(first_tile, cx.T)) {
+            stage_issue<S, C, K, CH, NT>(cx, first_tile, stage);
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            staged = true;
+        }
+        for (int m = 0; m < msub; m++) {
+            const int tile = first_tile + m;
+            if ((int64_t)tile * TILE >= cx.T) break;
+            const int nt = tile + 1;
+            const bool stage_next = USE_STAGE && m + 1 < msub && (int64_t)nt * TILE < cx.T &&
+                                    tile_is_interior<S, C, K, CH, NT>(nt, cx.T);
+            auto issue = [&]() {
+                if (stage_next) stage_issue<S, C, K, CH, NT>(cx, nt, stage);
+            };
+            if (USE_STAGE && staged)
+                pass_tile_1d<S, C, K, CH, NT, true, true>(cx, tile, lds_x, acc, stage, issue);
+            else if (tile_is_interior<S, C, K, CH, NT>(tile, cx.T))
+                pass_tile_1d<S, C, K, CH, NT, true, false>(cx, tile, lds_x, acc, stage, issue);
             else
-                pass_tile_1d<S, C, K, CH, NT, false>(A, inst, tile, lds_x, acc);
+                pass_tile_1d<S, C, K, CH, NT, false, false>(cx, tile, lds_x, acc, stage, issue);
+            if (stage_next) {
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
+            }
+            staged = stage_next;
         }
         block_reduce_part<NT>(acc, lds_red);
         if (threadIdx.x == 0) {
-            double *o = A.part + (int64_t)(D.tile0 + slot) * NPART;
+            double *o = A.part + (int64_t)slot_out * NPART;
 #pragma unroll
             for (int i = 0; i < NPART; i++) o[i] = acc[i];
         }

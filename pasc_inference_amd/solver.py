@@ -163,10 +163,15 @@ class Problem:
         check(lib().pasc_set_profiling(self.h, int(enable)))
 
     def autotune(self, cache_path: str | None = None):
-        b, c, us = C.c_int32(), C.c_int32(), C.c_double()
+        """Returns (block, chunk, prefetch, us_per_pass, max_slots) of the chosen launch shape."""
+        sh = capi.LaunchShape()
         check(lib().pasc_autotune(self.h, cache_path.encode() if cache_path else None,
-                                  C.byref(b), C.byref(c), C.byref(us)))
-        return b.value, c.value, us.value
+                                  C.byref(sh)))
+        return sh.block, sh.chunk, sh.prefetch, sh.us_per_pass, sh.max_slots
+
+    def set_launch_shape(self, block: int, chunk: int, prefetch: int = 0, max_slots: int = 0):
+        sh = capi.LaunchShape(block, chunk, prefetch, max_slots, 0.0)
+        check(lib().pasc_set_launch_shape(self.h, C.byref(sh)))
 
 
 class Batch:

@@ -231,12 +231,45 @@ def test_autotune_keeps_iterate_and_results(ctx, oracle):
         P.set_theta(theta)
         g0 = oracle.project_simplex(np.random.default_rng(1).random(T * K), K)
         P.set_gamma(g0)
-        block, chunk, us = P.autotune()
-        print(f"prec={prec}: block={block} chunk={chunk} {us:.1f} us/pass")
-        assert block in (128, 256, 512) and us > 0
+        block, chunk, pf, us, _ = P.autotune()
+        print(f"prec={prec}: block={block} chunk={chunk} prefetch={pf} {us:.1f} us/pass")
+        assert block in (128, 256, 512) and pf in (0, 1) and us > 0
         np.testing.assert_allclose(P.get_gamma(), g0, atol=1e-7 if prec else 0)
-        assert P.autotune()[:2] == (block, chunk)  # cached per context
+        assert P.autotune()[:3] == (block, chunk, pf)  # cached per context
         P.set_gamma_uniform()
         r = P.gamma_solve(maxit=100000, stop_mask=capi.STOP_KKT, kkt_tol=1e-9)
         g = P.get_gamma()
         assert r.kkt <= 1e-9 and np.max(np.abs(g - ref)) <= 1e-6
+
+
+@pytest.mark.parametrize("prec_name", ["FP64", "FP32_BULK"])
+def test_launch_shapes_give_identical_iterates(ctx, prec_name):
+    """Prefetching the next tile changes only the load order: with the same block, chunk and
+    slot count the iterates are bit-identical.  max_slots = 3 makes every workgroup run many
+    consecutive tiles, so the pipelined loads and the interior / boundary tile switch are
+    exercised at a small T.  (Other slot counts change the order of the FP64 partial sums,
+    which the BB steps amplify along the trajectory: those runs are not compared mid-solve.)"""
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Problem
+    prec = getattr(capi, prec_name)
+    T, K, eps = 50000, 3, 20.0
+    x, _ = _signal(31, T, [0.0, 1.0, 3.0], 1e-3, 0.9)
+    theta = np.array([0.0, 1.0, 3.0])
+    chunks = (4, 8) if prec == capi.FP32_BULK else (2, 4)
+    P = Problem(ctx, x, K, precision=prec, epssqr=eps)
+    P.set_theta(theta)
+    for block in (128, 256):
+        for chunk in chunks:
+            outs = []
+            for pf in (0, 1):
+                for slots in (3, 0):
+                    P.set_launch_shape(block, chunk, pf, slots)
+                    P.set_gamma_uniform()
+                    r = P.gamma_solve(maxit=300, stop_mask=0, polish_maxit=50)
+                    outs.append((pf, slots, r.it, r.fx, P.get_gamma()))
+            print(block, chunk, [(o[0], o[1], o[2], o[3]) for o in outs])
+            # same slot count: prefetch on and off agree bit for bit
+            assert outs[0][2] == outs[2][2] and np.array_equal(outs[0][4], outs[2][4])
+            assert outs[1][2] == outs[3][2] and np.array_equal(outs[1][4], outs[3][4])
+    with pytest.raises(capi.PascError):
+        P.set_launch_shape(192, chunks[0])

@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--scan-maxit", type=int, default=20000,
                     help="iteration cap of every scan solve, identical on both paths")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=4)
+    ap.add_argument("--cpu-iters", type=int, default=24)
     ap.add_argument("--no-graph", action="store_true",
                     help="launch iterations one by one instead of replaying hipGraphs")
     ap.add_argument("--profile-run", action="store_true",
@@ -63,106 +63,121 @@ def c2_workload(args):
     return x, lab, np.array(c["levels"]), c["eps"]
 
 
+def _timed_solves(args, P, params, label):
+    """W untimed warmup solves, then K timed solves from gamma0 = 1/K (wall clock around each,
+    device synchronised on both sides); returns the times and the last result."""
+    for _ in range(args.warmup):
+        P.set_gamma_uniform()
+        P.gamma_solve(**params)
+    times, res = [], None
+    for _ in range(args.steps):
+        P.set_gamma_uniform()
+        P.ctx.synchronize()
+        t0 = time.perf_counter()
+        res = P.gamma_solve(**params)
+        P.ctx.synchronize()
+        times.append(time.perf_counter() - t0)
+        log(args, f"{label}: {times[-1]:.4f}s it={res.it} ({res.it_fp32}+{res.it_fp64}) "
+                  f"kkt={res.kkt:.2e}")
+    return times, res
+
+
+def _profiled_pass(P, params, f64):
+    """One more solve with hipEvents around every fused pass: average pass time (s)."""
+    P.set_profiling(True)
+    P.set_gamma_uniform()
+    r = P.gamma_solve(**params)
+    P.set_profiling(False)
+    if f64:
+        return r.pass_time_fp64_s / max(r.passes_fp64, 1), r.passes_fp64
+    return r.pass_time_fp32_s / max(r.passes_fp32, 1), r.passes_fp32
+
+
+def _errors(g, f, kkt, gref, fref, truth, T, K):
+    lab = g.reshape(T, K).argmax(1)
+    labref = gref.reshape(T, K).argmax(1)
+    return dict(gamma_maxabs=float(np.max(np.abs(g - gref))), f_rel=float(abs(f - fref) / abs(fref)),
+                label_agreement=float(np.mean(lab == labref)),
+                label_vs_truth=float(np.mean(lab == truth)), kkt_fp64=float(kkt))
+
+
 def run_c2(args, ctx):
     from pasc_inference_amd import capi
     from pasc_inference_amd.solver import Problem
     x, truth, theta, eps = c2_workload(args)
     T, K = args.T, theta.size
-    n = T * K
+    tune = {}
+    fast = dict(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt, check_every=32,
+                use_graph=0 if args.no_graph else 1, kkt_every=4)
+
+    # FP32-bulk path (FP32 storage and bulk arithmetic, FP64 reductions, FP64 polish)
     P32 = Problem(ctx, x, K, precision=capi.FP32_BULK, epssqr=eps)
     P32.set_theta(theta)
-    tune = {}
     if not args.no_autotune:
         P32.set_gamma_uniform()
         tune["fp32"] = P32.autotune(args.tune_cache)
         log(args, f"autotune fp32: block={tune['fp32'][0]} chunk={tune['fp32'][1]} "
                   f"prefetch={tune['fp32'][2]} slots={tune['fp32'][4]} {tune['fp32'][3]:.1f} us/pass")
-    params = dict(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt, check_every=32,
-                  use_graph=0 if args.no_graph else 1, kkt_every=4)
-    for _ in range(args.warmup):
-        P32.set_gamma_uniform()
-        P32.gamma_solve(**params)
-    times, res = [], None
-    for _ in range(args.steps):
-        P32.set_gamma_uniform()
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        res = P32.gamma_solve(**params)
-        ctx.synchronize()
-        times.append(time.perf_counter() - t0)
-        log(args, f"fp32-bulk solve: {times[-1]:.4f}s it={res.it_fp32}+{res.it_fp64} kkt={res.kkt:.2e}")
+    t32, r32 = _timed_solves(args, P32, fast, "fp32-bulk solve")
     g32 = P32.get_gamma()
     f32, kkt32 = P32.objective()
-    # one extra solve with hipEvents around every fused pass (kernel time for the roofline)
-    P32.set_profiling(True)
-    P32.set_gamma_uniform()
-    prof = P32.gamma_solve(**params)
-    P32.set_profiling(False)
+    pass32, n32 = _profiled_pass(P32, fast, False)
+    pass32_polish, _ = _profiled_pass(P32, fast, True)
     P32.close()
 
-    # plain FP64 per-solve path (no graph, host poll every iteration)
+    # plain FP64 per-solve path: default launch shape, host poll every iteration, no graph
     P64 = Problem(ctx, x, K, precision=capi.FP64, epssqr=eps)
     P64.set_theta(theta)
-    P64.set_gamma_uniform()
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    r64 = P64.gamma_solve(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt)
-    ctx.synchronize()
-    t64 = time.perf_counter() - t0
-    log(args, f"plain fp64 solve: {t64:.4f}s it={r64.it} kkt={r64.kkt:.2e}")
-    # FP64 path with the graph-captured iteration loop and the autotuned launch shape
+    plain = dict(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt)
+    tp64, r64 = _timed_solves(args, P64, plain, "plain fp64 solve")
+    # FP64 path: autotuned launch shape, graph-captured iteration loop
     if not args.no_autotune:
         P64.set_gamma_uniform()
         tune["fp64"] = P64.autotune(args.tune_cache)
         log(args, f"autotune fp64: block={tune['fp64'][0]} chunk={tune['fp64'][1]} "
                   f"prefetch={tune['fp64'][2]} slots={tune['fp64'][4]} {tune['fp64'][3]:.1f} us/pass")
-    P64.set_gamma_uniform()
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    r64g = P64.gamma_solve(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt,
-                           check_every=32, use_graph=0 if args.no_graph else 1, kkt_every=4)
-    ctx.synchronize()
-    t64g = time.perf_counter() - t0
-    log(args, f"fp64 graph solve: {t64g:.4f}s it={r64g.it} kkt={r64g.kkt:.2e}")
+    t64g, r64g = _timed_solves(args, P64, fast, "fp64 graph solve")
+    g64 = P64.get_gamma()
+    f64, kkt64 = P64.objective()
+    pass64, n64 = _profiled_pass(P64, fast, True)
     # FP64 reference solve (tight KKT) for the error statement
-    P64.set_profiling(True)
     P64.set_gamma_uniform()
     rref = P64.gamma_solve(maxit=50000, stop_mask=capi.STOP_KKT, kkt_tol=1e-10, check_every=32,
-                           use_graph=0)
+                           use_graph=0 if args.no_graph else 1)
     log(args, f"fp64 reference solve: it={rref.it} kkt={rref.kkt:.2e}")
-    P64.set_profiling(False)
     gref = P64.get_gamma()
     fref, kref = P64.objective()
     P64.close()
-    lab32 = g32.reshape(T, K).argmax(1)
-    labref = gref.reshape(T, K).argmax(1)
-    err = dict(
-        gamma_maxabs=float(np.max(np.abs(g32 - gref))),
-        f_rel=float(abs(f32 - fref) / abs(fref)),
-        label_agreement=float(np.mean(lab32 == labref)),
-        label_vs_truth=float(np.mean(lab32 == truth)),
-        kkt_fp64=float(kkt32),
-        ref_kkt=float(kref),
-        ref_it=int(rref.it),
-    )
-    # roofline of the dominant kernel: the fused FP32 pass
-    avg_pass = prof.pass_time_fp32_s / max(prof.passes_fp32, 1)
-    avg_pass64 = prof.pass_time_fp64_s / max(prof.passes_fp64, 1) if prof.passes_fp64 else None
-    alg_bytes = 6.0 * n * 4  # algorithmic bytes per SPG iteration, s = 4 (FP32)
-    own_bytes = 2.0 * n * 4 + T * 4  # what the fused pass must move: gamma in/out + data
-    achieved = alg_bytes / avg_pass / 1e9
-    traffic = None
+    err32 = _errors(g32, f32, kkt32, gref, fref, truth, T, K)
+    err64 = _errors(g64, f64, kkt64, gref, fref, truth, T, K)
+    traffic = {}
     tf = os.path.join(ROOT, "profiles", "traffic_fused_pass_fp32_T1e7_K3.json")
     if os.path.exists(tf):
         try:
-            traffic = json.load(open(tf)).get("bytes_per_launch")
+            tj = json.load(open(tf))
+            traffic = {"fp32": tj.get("bytes_per_launch"), "fp64": tj.get("fp64_bytes_per_launch")}
         except Exception:
-            traffic = None
-    c2 = dict(times=times, res=res, t64=t64, r64=r64, t64g=t64g, r64g=r64g, err=err, tune=tune, avg_pass=avg_pass,
-              avg_pass64=avg_pass64, achieved=achieved, own_bytes=own_bytes,
-              alg_bytes=alg_bytes, traffic=traffic, x=x, theta=theta, T=T, K=K,
-              ref_pass64=(rref.pass_time_fp64_s / max(rref.passes_fp64, 1)))
-    return c2
+            traffic = {}
+    return dict(T=T, K=K, x=x, theta=theta, tune=tune, t32=t32, r32=r32, tp64=tp64, r64=r64,
+                t64g=t64g, r64g=r64g, pass32=pass32, n32=n32, pass32_polish=pass32_polish,
+                pass64=pass64, n64=n64, err32=err32, err64=err64, ref_kkt=float(kref),
+                ref_it=int(rref.it), traffic=traffic)
+
+
+def _roofline(kernel, s, n, T, avg, traffic, copy_gbps):
+    """The dominant kernel against the HBM roofline: achieved = algorithmic bytes per launch
+    (6*n*s, the vectors the reference iteration moves) / average launch time."""
+    alg = 6.0 * n * s
+    own = (2.0 * n + T) * s  # gamma_{i-1} read, gamma_i written, data read
+    out = {"bound": "hbm", "achieved": alg / avg / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": alg / avg / 1e9 / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
+           "avg_launch_us": avg * 1e6, "algorithmic_bytes_per_launch": alg,
+           "own_min_bytes_per_launch": own,
+           "own_min_frac": own / avg / 1e9 / HBM_PEAK_GBS, "device_copy_gbps": copy_gbps}
+    if traffic:
+        out["traffic_gbps"] = traffic / avg / 1e9
+        out["traffic_frac"] = traffic / avg / 1e9 / HBM_PEAK_GBS
+    return out
 
 
 def run_scan(args, ctx):
@@ -267,7 +282,7 @@ def profile_run(args):
             if args.tune_cache:
                 P.autotune(args.tune_cache)
             P.set_profiling(True)
-            r = P.gamma_solve(maxit=64, stop_mask=0, polish_maxit=1, check_every=16)
+            r = P.gamma_solve(maxit=64, stop_mask=0, polish_maxit=1, check_every=32, kkt_every=4)
             P.close()
             t = r.pass_time_fp32_s if prec == capi.FP32_BULK else r.pass_time_fp64_s
             n = r.passes_fp32 if prec == capi.FP32_BULK else r.passes_fp64
@@ -291,62 +306,56 @@ def main():
         c2 = run_c2(args, ctx)
         scan = None if args.no_scan else run_scan(args, ctx)
     cpu = None if args.no_cpu else run_cpu_baseline(args, c2)
-    t32 = float(np.mean(c2["times"]))
-    best_path = "fp32-bulk + fp64 polish" if t32 <= c2["t64g"] else "fp64 (graph-captured)"
-    best = min(t32, c2["t64g"])
-    n = c2["T"] * c2["K"]
-    if t32 <= c2["t64g"]:
-        solve_bytes = 6.0 * n * (4 * c2["res"].it_fp32 + 8 * c2["res"].it_fp64)
+    t32, t64g = float(np.mean(c2["t32"])), float(np.mean(c2["t64g"]))
+    fp64_best = t64g < t32
+    best = t64g if fp64_best else t32
+    T, K = c2["T"], c2["K"]
+    n = T * K
+    r32, r64g = c2["r32"], c2["r64g"]
+    if fp64_best:
+        solve_bytes = 6.0 * n * 8 * r64g.it
+        roof = _roofline("spg_pass_1d<double,K=3> (fused SPG iteration, FP64)", 8, n, T,
+                         c2["pass64"], c2["traffic"].get("fp64"), copy_gbps)
+        other = _roofline("spg_pass_1d<float,K=3> (fused SPG iteration, FP32 phase)", 4, n, T,
+                          c2["pass32"], c2["traffic"].get("fp32"), copy_gbps)
     else:
-        solve_bytes = 6.0 * n * 8 * c2["r64g"].it
-    ms = best * 1e3
-    r = c2["res"]
+        solve_bytes = 6.0 * n * (4 * r32.it_fp32 + 8 * r32.it_fp64)
+        roof = _roofline("spg_pass_1d<float,K=3> (fused SPG iteration, FP32 phase)", 4, n, T,
+                         c2["pass32"], c2["traffic"].get("fp32"), copy_gbps)
+        other = _roofline("spg_pass_1d<double,K=3> (fused SPG iteration, FP64)", 8, n, T,
+                          c2["pass64"], c2["traffic"].get("fp64"), copy_gbps)
     line = {
         "metric": "time-to-KKT 1e-6 at T=1e7 K=3; batched eps-scan solves/s vs plain, same GPU",
         "value": best,
-        "path": best_path,
-        "fp32_bulk_time_to_kkt_s": t32,
+        "path": "fp64 (autotuned, graph-captured)" if fp64_best else "fp32-bulk + fp64 polish",
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": best * 1e3,
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f32 bulk + f64 reductions/polish",
+        "dtype": "f64" if fp64_best else "f32 bulk + f64 reductions/polish",
         "data": "synthetic (deterministic SplitMix64 Markov-chain signal generated on the host)",
-        "config": {"workload": f"gamma QP solve, T={c2['T']}, K={c2['K']}, levels {{0,1,3}}, "
+        "config": {"workload": f"gamma QP solve, T={T}, K={K}, levels {{0,1,3}}, "
                                f"switch 1e-4, sigma 1, seed 1, epsilon 100, theta fixed, "
                                f"gamma0=1/K, KKT<={args.kkt:g}",
-                   "T": c2["T"], "K": c2["K"], "epssqr": 100.0, "kkt_tol": args.kkt},
-        "iterations": {"fp32": r.it_fp32, "fp64_polish": r.it_fp64,
+                   "T": T, "K": K, "epssqr": 100.0, "kkt_tol": args.kkt},
+        "paths_s": {"fp64_graph": t64g, "fp32_bulk": t32, "plain_fp64": float(np.mean(c2["tp64"]))},
+        "iterations": {"fp64_graph": r64g.it, "fp32_bulk": [r32.it_fp32, r32.it_fp64],
                        "plain_fp64": c2["r64"].it},
-        "plain_fp64_time_to_kkt_s": c2["t64"],
-        "fp64_graph_time_to_kkt_s": c2["t64g"],
-        "fp64_graph_iterations": c2["r64g"].it,
+        "speedup_vs_plain_fp64": float(np.mean(c2["tp64"])) / best,
         "autotune": {k: {"block": v[0], "chunk": v[1], "prefetch": v[2], "us_per_pass": v[3],
                          "max_slots": v[4]}
                      for k, v in c2["tune"].items()},
-        "speedup_vs_plain_fp64": c2["t64"] / best,
         # whole solve against the roofline, 6*n*s algorithmic bytes per SPG iteration
         "solve_roofline_frac": solve_bytes / best / (HBM_PEAK_GBS * 1e9),
-        "errors_vs_fp64_reference_solve": c2["err"],
-        "roofline": {
-            "bound": "hbm",
-            "achieved": c2["achieved"],
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": c2["achieved"] / HBM_PEAK_GBS,
-            "traffic": c2["traffic"],
-            "kernel": "spg_pass_1d<float,K=3> (fused SPG iteration)",
-            "avg_launch_us": c2["avg_pass"] * 1e6,
-            "algorithmic_bytes_per_launch": c2["alg_bytes"],
-            "own_min_bytes_per_launch": c2["own_bytes"],
-            "own_min_frac": c2["own_bytes"] / c2["avg_pass"] / 1e9 / HBM_PEAK_GBS,
-            "fp64_pass_avg_us": (c2["ref_pass64"] * 1e6) if c2["ref_pass64"] else None,
-            "device_copy_gbps": copy_gbps,
-        },
+        "errors_vs_fp64_reference_solve": {"fp64_graph": c2["err64"], "fp32_bulk": c2["err32"],
+                                           "ref_kkt": c2["ref_kkt"], "ref_it": c2["ref_it"]},
+        "roofline": roof,
+        "roofline_other_path": other,
+        "fp32_polish_pass_avg_us": c2["pass32_polish"] * 1e6,
         "scan": scan,
         "cpu_baseline": cpu,
     }

@@ -72,7 +72,8 @@ typedef struct {
     int32_t xdim;              /* data dimension (reference: 1; xdim > 1 is an extension) */
     int32_t scalef;            /* graphh1femmodel_scalef (default 1): A = eps/T*A0, b /= T */
     int32_t precision;         /* pasc_precision */
-    const double *x;           /* host data, [T][R][xdim] */
+    const double *x;           /* host data, [T][R][xdim]; NULL: no data term -- the QP is then
+                                  defined by an explicit b (pasc_set_linear_term) only */
     const int64_t *nbr_ptr;    /* optional CSR adjacency of the R nodes (R+1 entries) */
     const int32_t *nbr_idx;    /* neighbour ids */
     const uint8_t *heldout;    /* optional [T*R] mask: held-out nodes carry no data term */

@@ -53,6 +53,7 @@ struct pasc_problem {
     bool gamma_set = false;
     bool solved = false;
     bool b_set = false;  // explicit linear term instead of the data term
+    bool has_data = true;  // false: created without x (only an explicit b can define the QP)
     uint8_t *d_heldout = nullptr;
     std::vector<uint8_t> h_heldout;
     std::vector<double> h_x;  // host copy of the data, [T][R][xdim] (for abserr etc.)
@@ -192,7 +193,6 @@ int pasc_problem_create(pasc_ctx *ctx, const pasc_problem_desc *desc, pasc_probl
     if (desc->T < 1 || desc->R < 1) return fail(PASC_ERR_ARG, "T and R must be >= 1");
     if (desc->K < 1 || desc->K > PASC_MAX_K) return fail(PASC_ERR_ARG, "K must be in [1, %d]", PASC_MAX_K);
     if (desc->xdim < 1 || desc->xdim > 3) return fail(PASC_ERR_ARG, "xdim must be in [1, 3]");
-    if (!desc->x) return fail(PASC_ERR_ARG, "x is NULL");
     if (desc->precision != PASC_FP64 && desc->precision != PASC_FP32_BULK)
         return fail(PASC_ERR_ARG, "unknown precision %d", desc->precision);
     const bool grid = desc->grid_width > 0;
@@ -217,7 +217,8 @@ int pasc_problem_create(pasc_ctx *ctx, const pasc_problem_desc *desc, pasc_probl
     p->precision = desc->precision;
     p->grid_width = desc->grid_width;
     const int64_t n = grid ? p->R : p->T;
-    p->h_x.assign(desc->x, desc->x + n * p->xdim);
+    p->has_data = desc->x != nullptr;
+    if (p->has_data) p->h_x.assign(desc->x, desc->x + n * p->xdim);
     if (desc->heldout) {
         p->h_heldout.assign(desc->heldout, desc->heldout + n);
         hipError_t e = hipMalloc(&p->d_heldout, (size_t)n);
@@ -238,7 +239,8 @@ int pasc_problem_create(pasc_ctx *ctx, const pasc_problem_desc *desc, pasc_probl
         // data rows: SoA, row c holds channel c
         std::vector<double> rows((size_t)d.ld * p->xdim, 0.0);
         for (int c = 0; c < p->xdim; c++)
-            for (int64_t i = 0; i < n; i++) rows[(size_t)c * d.ld + i] = desc->x[i * p->xdim + c];
+            for (int64_t i = 0; p->has_data && i < n; i++)
+                rows[(size_t)c * d.ld + i] = desc->x[i * p->xdim + c];
         rc = engine_upload_data(p->e, rows.data(), 0, (int64_t)rows.size());
     }
     if (rc) {
@@ -329,6 +331,8 @@ int pasc_gamma_solve(pasc_problem *p, const pasc_spg_params *params, pasc_spg_re
     if (rc) return rc;
     if (!p->theta_set && !p->b_set)
         return fail(PASC_ERR_STATE, "gamma_solve: neither theta nor an explicit b was set");
+    if (!p->has_data && !p->b_set)
+        return fail(PASC_ERR_STATE, "gamma_solve: the problem has no data; set an explicit b");
     rc = use_device(p->ctx);
     if (rc) return rc;
     if (!p->gamma_set) {
@@ -367,6 +371,7 @@ int pasc_theta_update(pasc_problem *p) {
     clear_error();
     if (!p) return fail(PASC_ERR_ARG, "theta_update: NULL problem");
     if (!p->gamma_set) return fail(PASC_ERR_STATE, "theta_update: gamma was never set");
+    if (!p->has_data) return fail(PASC_ERR_STATE, "theta_update: the problem has no data");
     int rc = use_device(p->ctx);
     if (rc) return rc;
     rc = engine_theta_update(p->e, 0, nullptr);
@@ -379,6 +384,7 @@ int pasc_ts_solve(pasc_problem *p, const pasc_ts_params *tp, const pasc_spg_para
     clear_error();
     if (!p || !tp) return fail(PASC_ERR_ARG, "ts_solve: NULL argument");
     if (p->b_set) return fail(PASC_ERR_STATE, "ts_solve: the problem has an explicit b (theta would not enter it)");
+    if (!p->has_data) return fail(PASC_ERR_STATE, "ts_solve: the problem has no data");
     int rc = check_params(sp);
     if (rc) return rc;
     rc = use_device(p->ctx);
@@ -574,7 +580,7 @@ int pasc_adjoint(pasc_problem *p, const uint8_t *mask, double cg_tol, int32_t cg
     if (!p || !mask || !loss || !dlogeps || !dtheta) return fail(PASC_ERR_ARG, "adjoint: NULL argument");
     if (p->xdim != 1 || p->grid_width > 0) return fail(PASC_ERR_ARG, "adjoint: 1-D signals (xdim == 1) only");
     if (!p->solved) return fail(PASC_ERR_STATE, "adjoint: solve the gamma problem first");
-    if (p->b_set) return fail(PASC_ERR_STATE, "adjoint: the problem has an explicit b");
+    if (p->b_set || !p->has_data) return fail(PASC_ERR_STATE, "adjoint: needs the data term (no explicit b)");
     int rc = use_device(p->ctx);
     if (rc) return rc;
     int it = 0;

@@ -49,8 +49,14 @@ class Problem:
                  scalef: bool = True, precision: int = capi.FP64, heldout=None,
                  grid_width: int = 0, epssqr: float = 1.0):
         self.ctx = ctx
-        self.x = np.ascontiguousarray(x, dtype=np.float64).ravel()
-        n = self.x.size // xdim
+        if x is None:  # no data term: the QP is defined by an explicit b
+            if T is None:
+                raise ValueError("x=None needs T (and R / grid_width)")
+            self.x = None
+            n = T * R
+        else:
+            self.x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+            n = self.x.size // xdim
         if T is None:
             T = n if grid_width == 0 else 1
         self.T, self.R, self.K, self.xdim = T, R if grid_width == 0 else n, K, xdim
@@ -60,7 +66,7 @@ class Problem:
         d = capi.ProblemDesc()
         d.T, d.R, d.K, d.xdim = self.T, self.R, K, xdim
         d.scalef, d.precision = int(scalef), precision
-        d.x = dptr(self.x)
+        d.x = dptr(self.x) if self.x is not None else None
         d.heldout = u8ptr(self.heldout) if self.heldout is not None else None
         d.grid_width = grid_width
         h = C.c_void_p()

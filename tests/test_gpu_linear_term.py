@@ -101,3 +101,31 @@ def test_explicit_b_on_grid(ctx, oracle):
         assert kkt <= 1e-11 * 1.001
         assert np.max(np.abs(g - ref)) <= 1e-6
         assert abs(f - rr.fx) <= 1e-8 * abs(rr.fx)
+
+
+def test_problem_without_data_is_defined_by_b(ctx, oracle):
+    """The SPG-level drop-in: no data vector at all, A = coeff*A0 and b as QPData holds them
+    (coeff = epssqr/T with scalef, so epssqr = coeff*T keeps the per-sample KKT scale)."""
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Problem
+    T, K, coeff = 9000, 3, 4.0e-3
+    b = _rand_b(5, T, K, 1.0 / T).ravel()
+    ref, rr = oracle.spg_solve(np.full(T * K, 1.0 / K), b, T, 1, K, coeff, maxit=400000,
+                               stop_difff=0, stop_kkt=1, kkt_tol=1e-12, kkt_scale=T, nthreads=8,
+                               gd_centered=1)
+    P = Problem(ctx, None, K, T=T, precision=capi.FP64, epssqr=coeff * T)
+    P.set_gamma_uniform()
+    with pytest.raises(capi.PascError):
+        P.gamma_solve(maxit=10)  # no data and no b
+    with pytest.raises(capi.PascError):
+        P.theta_update()
+    P.set_linear_term(b)
+    r = P.gamma_solve(maxit=400000, stop_mask=capi.STOP_KKT, kkt_tol=1e-11)
+    g = P.get_gamma()
+    print(f"no-data problem: it={r.it} kkt={r.kkt:.2e} err={np.max(np.abs(g - ref)):.2e}", flush=True)
+    assert r.stop_reason == capi.REASON_KKT
+    assert np.max(np.abs(g - ref)) <= 1e-6 and abs(r.fx - rr.fx) <= 1e-8 * abs(rr.fx)
+    # the reference's own default stop (|f - f_old| < 1e-9) through the same entry point
+    P.set_gamma_uniform()
+    r2 = P.gamma_solve(maxit=100000)
+    assert r2.stop_reason == capi.REASONS.index("difff") and r2.it > 0

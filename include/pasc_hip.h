@@ -79,9 +79,16 @@ typedef struct {
     const uint8_t *heldout;    /* optional [T*R] mask: held-out nodes carry no data term */
     int32_t grid_width;        /* > 0: the graph is the 4-neighbour grid of width grid_width
                                   (R = width*height); nbr_ptr/nbr_idx may then be NULL */
+    double fem_reduce;         /* graphh1femmodel_fem_reduce: in (0, 1) the gamma QP is solved on
+                                  a coarse time grid of T2 = ceil(T*fem_reduce) steps
+                                  (graphh1fem.cpp:115-214,236-301); 0 or >= 1: no reduction */
+    int32_t fem_type;          /* PASC_FEM_CONSTANT (Fem) or PASC_FEM_HAT (FemHat); 1-D only */
 } pasc_problem_desc;
 
 #define PASC_MAX_K 8
+
+/* FEM coarsening of the time axis (graphh1femmodel_fem_type / fem_reduce) */
+enum { PASC_FEM_NONE = 0, PASC_FEM_CONSTANT = 1, PASC_FEM_HAT = 2 };
 #define PASC_MAX_M 64
 
 /* SPG-QP settings (spgqpsolver.h:19-33, set_settings_from_console :161-202) */
@@ -211,6 +218,17 @@ int pasc_project_simplex(pasc_ctx *ctx, void *x_dev, int64_t nblocks, int32_t K,
  * x, y are k-major device arrays with row stride ld. */
 int pasc_h1_matvec(pasc_ctx *ctx, const void *x_dev, void *y_dev, int64_t T, int64_t R,
                    int32_t K, int64_t ld, double alpha, int32_t grid_width, int dtype);
+
+/* ---- FEM coarsening, kernel level (caller-owned device memory, k-major FP64 rows) ----
+ * Fem<PetscVector>::reduce_gamma / prolongate_gamma (fem.cpp:28-166) for PASC_FEM_CONSTANT and
+ * FemHat<PetscVector>::reduce_gamma / prolongate_gamma (femhat.cpp:26-217) for PASC_FEM_HAT,
+ * one rank: fine rows in[k*ld1 + t1] (t1 < T1) <-> coarse rows out[k*ld2 + t2] (t2 < T2).
+ * pasc_fem_reduced_T gives T2 = ceil(T1*fem_reduce) (fem.cpp:198), T1 for fem_reduce >= 1. */
+int64_t pasc_fem_reduced_T(int64_t T1, double fem_reduce);
+int pasc_fem_reduce(pasc_ctx *ctx, const double *fine_dev, double *coarse_dev, int64_t T1,
+                    int64_t T2, int32_t K, int64_t ld1, int64_t ld2, int32_t fem_type);
+int pasc_fem_prolongate(pasc_ctx *ctx, const double *coarse_dev, double *fine_dev, int64_t T1,
+                        int64_t T2, int32_t K, int64_t ld1, int64_t ld2, int32_t fem_type);
 
 /* ---- batched epsilon scan (new capability; cf. the sequential scan test_signal1D.cpp:268-353) --
  * n_series series (each its own T, K, data and theta), each solved for every epsilon in eps[]:

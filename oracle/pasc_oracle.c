@@ -439,10 +439,78 @@ double or_objective(const double *x, const double *b, int64_t T, int64_t R, int 
 /* outer alternation                                                                          */
 /* ------------------------------------------------------------------------------------------ */
 
+static int or_ts_solve_fem(const double *x, double *gamma, double *theta, int64_t T, int K,
+                           int xdim, double epssqr, const uint8_t *heldout,
+                           const or_ts_params *tp, const or_spg_params *sp, or_ts_result *res);
+
+int64_t or_fem_reduced_T(int64_t T1, double fem_reduce) {
+    /* fem.cpp:198 / femhat.cpp:224: T_reduced = ceil(T * fem_reduce) when reduced */
+    if (fem_reduce >= 1.0) return T1;
+    return (int64_t)ceil((double)T1 * fem_reduce);
+}
+
+void or_fem_reduce(const double *in, double *out, int64_t T1, int64_t T2, int K, int type) {
+    if (type == 1) {
+        const double diff = (double)T1 / (double)T2;            /* fem.cpp:16 */
+        for (int k = 0; k < K; k++)
+            for (int64_t t2 = 0; t2 < T2; t2++) {
+                double s = 0.0;
+                for (int64_t i = (int64_t)round((double)t2 * diff);
+                     i < (int64_t)round((double)(t2 + 1) * diff); i++)
+                    s += in[i * K + k];
+                out[t2 * K + k] = s;
+            }
+        return;
+    }
+    const double diff = (double)(T1 - 1) / (double)(T2 - 1);    /* femhat.cpp:15 */
+    const int64_t right_idx = (int64_t)floor(diff * (double)(T2 - 1));  /* femhat.h:179 */
+    for (int k = 0; k < K; k++)
+        for (int64_t t2 = 0; t2 < T2; t2++) {
+            const double center = (double)t2 * diff;
+            const double left = (double)(t2 - 1) * diff;
+            const double right = (double)(t2 + 1) * diff;
+            double s = 0.0;
+            int64_t t1 = (int64_t)floor(left);
+            for (; (double)t1 <= center; t1++)
+                if (t1 >= 0) s += ((double)t1 - left) / (center - left) * in[t1 * K + k];
+            for (; (double)t1 < right; t1++)
+                if (t1 < right_idx) s += ((double)t1 - right) / (center - right) * in[t1 * K + k];
+            out[t2 * K + k] = s;
+        }
+}
+
+void or_fem_prolongate(const double *in, double *out, int64_t T1, int64_t T2, int K, int type) {
+    if (type == 1) {
+        const double diff = (double)T1 / (double)T2;
+        for (int k = 0; k < K; k++)
+            for (int64_t t2 = 0; t2 < T2; t2++)
+                for (int64_t i = (int64_t)round((double)t2 * diff);
+                     i < (int64_t)round((double)(t2 + 1) * diff); i++)
+                    out[i * K + k] = in[t2 * K + k];
+        return;
+    }
+    const double diff = (double)(T1 - 1) / (double)(T2 - 1);
+    for (int k = 0; k < K; k++)
+        for (int64_t t1 = 0; t1 < T1; t1++) {
+            const int64_t tl = (int64_t)floor((double)t1 / diff);
+            const int64_t tr = tl + 1;
+            const double t1l = (double)tl * diff, t1r = (double)tr * diff;
+            const double vr = in[(tr < T2 ? tr : T2 - 1) * K + k];
+            const double vl = in[(tl < T2 ? tl : T2 - 1) * K + k];
+            out[t1 * K + k] = ((double)t1 - t1l) / (t1r - t1l) * vr +
+                              ((double)t1 - t1r) / (t1l - t1r) * vl;
+        }
+}
+
 int or_ts_solve(const double *x, double *gamma, double *theta, int64_t T, int64_t R, int K,
                 int xdim, double epssqr, const int64_t *nbr_ptr, const int32_t *nbr_idx,
                 const uint8_t *heldout, const or_ts_params *tp, const or_spg_params *sp,
                 or_ts_result *res) {
+    const int fem = tp->fem_reduce > 0.0 && tp->fem_reduce < 1.0 && tp->fem_type > 0;
+    if (fem) {
+        if (R != 1) return -1;
+        return or_ts_solve_fem(x, gamma, theta, T, K, xdim, epssqr, heldout, tp, sp, res);
+    }
     const int nthreads = fix_threads(sp->nthreads);
     const int64_t n = T * R * K;
     double *b = (double *)malloc(sizeof(double) * (size_t)n);
@@ -516,4 +584,63 @@ double or_host_copy_bandwidth(int64_t nbytes, int reps, int nthreads) {
     free(a);
     free(c);
     return 2.0 * (double)n * 8.0 / best;
+}
+
+/* TSSolver alternation with FEM coarsening (graphh1fem.cpp:115-214,236-301): the residual
+ * (x_t - theta_k)^2 (graphh1fem.cpp:261) and the fine gamma are reduced to the coarse grid
+ * (:274-277), b = -reduced residual / T2 with scalef (:281-286), A = epssqr/T * A0 on the coarse
+ * grid with scalef, epssqr*T2/T without (:146-151), the coarse solution is prolongated back
+ * (:296-298); theta is updated from the fine gamma.  Held-out steps carry no residual. */
+static int or_ts_solve_fem(const double *x, double *gamma, double *theta, int64_t T, int K,
+                           int xdim, double epssqr, const uint8_t *heldout,
+                           const or_ts_params *tp, const or_spg_params *sp, or_ts_result *res) {
+    const int nthreads = fix_threads(sp->nthreads);
+    const int64_t T2 = or_fem_reduced_T(T, tp->fem_reduce);
+    if (tp->fem_type == 2 && T2 < 2) return -1;
+    double *resid = (double *)malloc(sizeof(double) * (size_t)(T * K));
+    double *b2 = (double *)malloc(sizeof(double) * (size_t)(T2 * K));
+    double *g2 = (double *)malloc(sizeof(double) * (size_t)(T2 * K));
+    if (!resid || !b2 || !g2) { free(resid); free(b2); free(g2); return -1; }
+    const double alpha = tp->scalef ? epssqr / (double)T : epssqr * (double)T2 / (double)T;
+    const double bs = tp->scalef ? -1.0 / (double)T2 : -1.0;
+    double L = DBL_MAX, L_old, deltaL = DBL_MAX;
+    int it, spg_sum = 0, increased = 0;
+    for (it = 0; it < tp->maxit; it++) {
+        if (!tp->theta_fixed && !(it == 0 && tp->skip_first_theta))
+            or_theta_update(x, gamma, theta, T, 1, K, xdim, heldout, nthreads);
+        for (int64_t t = 0; t < T; t++)
+            for (int k = 0; k < K; k++) {
+                double q = 0.0;
+                if (!(heldout && heldout[t]))
+                    for (int c = 0; c < xdim; c++) {
+                        const double dx = x[t * xdim + c] - theta[k * xdim + c];
+                        q += dx * dx;
+                    }
+                resid[t * K + k] = q;
+            }
+        or_fem_reduce(resid, b2, T, T2, K, tp->fem_type);
+        for (int64_t i = 0; i < T2 * K; i++) b2[i] *= bs;
+        or_fem_reduce(gamma, g2, T, T2, K, tp->fem_type);
+        or_spg_result r;
+        if (or_spg_solve(g2, b2, T2, 1, K, alpha, NULL, NULL, sp, &r) != 0) {
+            free(resid); free(b2); free(g2);
+            return -1;
+        }
+        or_fem_prolongate(g2, gamma, T, T2, K, tp->fem_type);
+        spg_sum += r.it;
+        L_old = L;
+        L = r.fx;
+        deltaL = fabs(L - L_old);
+        if (L - L_old > 0) increased++;
+        if (deltaL < tp->eps) { it++; break; }
+    }
+    if (res) {
+        res->it = it;
+        res->spg_it_sum = spg_sum;
+        res->L = L;
+        res->deltaL = deltaL;
+        res->increased = increased;
+    }
+    free(resid); free(b2); free(g2);
+    return 0;
 }

@@ -103,6 +103,23 @@ int or_spg_solve(double *x, const double *b, int64_t T, int64_t R, int K, double
                  const int64_t *nbr_ptr, const int32_t *nbr_idx, const or_spg_params *p,
                  or_spg_result *res);
 
+/* ---- FEM coarsening of the time axis (fem.cpp:28-182, femhat.cpp:26-217), one MPI rank ----
+ * Arrays are t-major with K innermost, as gamma: in[t1*K + k] on the fine grid of T1 steps,
+ * out[t2*K + k] on the coarse grid of T2 = ceil(T1 * fem_reduce) steps
+ * (fem.cpp:198, femhat.cpp:224).
+ * type 1 (Fem, piecewise constant): diff = T1/T2; coarse cell t2 holds the fine steps
+ *   round(t2*diff) .. round((t2+1)*diff)-1; reduce sums over the cell (fem.cpp:76-83),
+ *   prolongate copies the coarse value into the cell (fem.cpp:148-153).
+ * type 2 (FemHat, hat functions): diff = (T1-1)/(T2-1); reduce is the hat-weighted sum of
+ *   femhat.cpp:68-101 as written -- the left loop starts at floor(left), so for a non-integer
+ *   left the step floor(left) enters with the negative weight (floor(left)-left)/diff, and the
+ *   right loop stops before floor(diff*(T2-1)) (femhat.h:154-180 with one rank); prolongate is
+ *   the linear interpolation of femhat.cpp:156-175 (a right neighbour index of T2, whose
+ *   weight is 0, reads the last coarse value). */
+int64_t or_fem_reduced_T(int64_t T1, double fem_reduce);
+void or_fem_reduce(const double *in, double *out, int64_t T1, int64_t T2, int K, int type);
+void or_fem_prolongate(const double *in, double *out, int64_t T1, int64_t T2, int K, int type);
+
 /* ---- outer alternation: tssolver.h:491-715, model hooks graphh1fem.cpp ---- */
 typedef struct {
     int maxit;            /* tssolver_maxit, default 300 */
@@ -110,6 +127,8 @@ typedef struct {
     int theta_fixed;      /* set_solution_theta (tssolver.h:777-790): never update theta */
     int skip_first_theta; /* start with a gamma step from the given theta */
     int scalef;           /* graphh1femmodel_scalef, default 1 */
+    double fem_reduce;    /* graphh1femmodel_fem_reduce (< 1: gamma solved on a coarse grid) */
+    int fem_type;         /* 1 Fem, 2 FemHat (used when fem_reduce < 1; R must be 1) */
 } or_ts_params;
 
 typedef struct {

@@ -44,7 +44,8 @@ class SpgResult(C.Structure):
 class TsParams(C.Structure):
     _fields_ = [
         ("maxit", C.c_int), ("eps", C.c_double), ("theta_fixed", C.c_int),
-        ("skip_first_theta", C.c_int), ("scalef", C.c_int),
+        ("skip_first_theta", C.c_int), ("scalef", C.c_int), ("fem_reduce", C.c_double),
+        ("fem_type", C.c_int),
     ]
 
 
@@ -100,6 +101,10 @@ def lib():
         L.or_host_copy_bandwidth.argtypes = [C.c_int64, C.c_int, C.c_int]
         L.or_host_copy_bandwidth.restype = C.c_double
         L.or_max_threads.restype = C.c_int
+        L.or_fem_reduced_T.argtypes = [C.c_int64, C.c_double]
+        L.or_fem_reduced_T.restype = C.c_int64
+        L.or_fem_reduce.argtypes = [_f64, _f64, C.c_int64, C.c_int64, C.c_int, C.c_int]
+        L.or_fem_prolongate.argtypes = [_f64, _f64, C.c_int64, C.c_int64, C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -217,7 +222,8 @@ def ts_solve(x, gamma0, theta0, T: int, R: int, K: int, xdim: int, epssqr: float
     x = np.ascontiguousarray(x, dtype=np.float64).ravel()
     gamma = np.array(gamma0, dtype=np.float64).ravel().copy()
     theta = np.array(theta0, dtype=np.float64).ravel().copy()
-    tp = TsParams(maxit=300, eps=1e-6, theta_fixed=0, skip_first_theta=0, scalef=1)
+    tp = TsParams(maxit=300, eps=1e-6, theta_fixed=0, skip_first_theta=0, scalef=1,
+                  fem_reduce=1.0, fem_type=0)
     for k, v in (ts or {}).items():
         setattr(tp, k, v)
     sp = spg_params(**spg)
@@ -241,3 +247,26 @@ def abserr_reconstructed(gamma, theta, solution, T: int, R: int, K: int) -> floa
 
 def host_copy_bandwidth(nbytes: int = 1 << 28, reps: int = 3, nthreads: int = 1) -> float:
     return float(lib().or_host_copy_bandwidth(nbytes, reps, nthreads))
+
+
+FEM_CONSTANT, FEM_HAT = 1, 2
+
+
+def fem_reduced_T(T1: int, fem_reduce: float) -> int:
+    return int(lib().or_fem_reduced_T(T1, fem_reduce))
+
+
+def fem_reduce(x, T1: int, T2: int, K: int, fem_type: int):
+    """Fine -> coarse (fem.cpp:28-95 / femhat.cpp:26-114), t-major [T*K] arrays."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    out = np.zeros(T2 * K)
+    lib().or_fem_reduce(x, out, T1, T2, K, fem_type)
+    return out
+
+
+def fem_prolongate(x, T1: int, T2: int, K: int, fem_type: int):
+    """Coarse -> fine (fem.cpp:97-166 / femhat.cpp:116-217), t-major [T*K] arrays."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    out = np.zeros(T1 * K)
+    lib().or_fem_prolongate(x, out, T1, T2, K, fem_type)
+    return out

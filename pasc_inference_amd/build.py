@@ -34,6 +34,7 @@ def _units():
         units.append((os.path.join(CSRC, "grid_inst.hip"), f"grid_{s}.o", [f"-DPASS_S={s}"]))
     units.append((os.path.join(CSRC, "engine.hip"), "engine.o", []))
     units.append((os.path.join(CSRC, "adjoint.hip"), "adjoint.o", []))
+    units.append((os.path.join(CSRC, "fem.hip"), "fem.o", []))
     units.append((os.path.join(CSRC, "capi.hip"), "capi.o", []))
     units.append((os.path.join(CSRC, "generate.cpp"), "generate.o", []))
     return units

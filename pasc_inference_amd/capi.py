@@ -17,6 +17,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "pasc_hip.h")
 PASC_OK, PASC_ERR_ARG, PASC_ERR_HIP, PASC_ERR_ALLOC, PASC_ERR_STATE, PASC_ERR_NUMERIC = range(6)
 LAYOUT_TMAJOR, LAYOUT_KMAJOR = 0, 1
 FP64, FP32_BULK = 0, 1
+FEM_NONE, FEM_CONSTANT, FEM_HAT = 0, 1, 2
 DTYPE_F64, DTYPE_F32 = 0, 1
 STOP_DIFFF, STOP_NORMGP, STOP_NORMGP_NORMB, STOP_ANORMGP, STOP_ANORMGP_NORMB, STOP_KKT = (
     1, 2, 4, 8, 16, 32)
@@ -38,6 +39,7 @@ class ProblemDesc(C.Structure):
         ("scalef", C.c_int32), ("precision", C.c_int32), ("x", C.POINTER(C.c_double)),
         ("nbr_ptr", C.POINTER(C.c_int64)), ("nbr_idx", C.POINTER(C.c_int32)),
         ("heldout", C.POINTER(C.c_uint8)), ("grid_width", C.c_int32),
+        ("fem_reduce", C.c_double), ("fem_type", C.c_int32),
     ]
 
 
@@ -107,6 +109,11 @@ _SIGS = {
     "pasc_gamma_solve": ([vp, C.POINTER(SpgParams), C.POINTER(SpgResult)], C.c_int),
     "pasc_theta_update": ([vp], C.c_int),
     "pasc_set_linear_term": ([vp, C.POINTER(C.c_double), C.c_int], C.c_int),
+    "pasc_fem_reduced_T": ([C.c_int64, C.c_double], C.c_int64),
+    "pasc_fem_reduce": ([vp, vp, vp, C.c_int64, C.c_int64, C.c_int32, C.c_int64, C.c_int64,
+                         C.c_int32], C.c_int),
+    "pasc_fem_prolongate": ([vp, vp, vp, C.c_int64, C.c_int64, C.c_int32, C.c_int64, C.c_int64,
+                             C.c_int32], C.c_int),
     "pasc_ts_default_params": ([C.POINTER(TsParams)], None),
     "pasc_ts_solve": ([vp, C.POINTER(TsParams), C.POINTER(SpgParams), C.POINTER(TsResult)],
                       C.c_int),

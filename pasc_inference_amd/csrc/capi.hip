@@ -58,6 +58,14 @@ struct pasc_problem {
     std::vector<uint8_t> h_heldout;
     std::vector<double> h_x;  // host copy of the data, [T][R][xdim] (for abserr etc.)
     Engine e;
+    // FEM coarsening (1-D): the engine holds the coarse QP of T2 steps, the fine grid of T
+    // steps lives here (data, gamma) and meets the engine through reduce / prolongate
+    int fem_type = PASC_FEM_NONE;
+    FemGeom fg{};
+    double *d_x1 = nullptr;    // fine data rows [xdim][ld1]
+    double *d_g1 = nullptr;    // fine gamma rows [K][ld1]
+    double *d_tmp = nullptr;   // coarse rows [K][ld2]
+    InstDesc fine{};           // descriptor of the fine rows (theta step, labels, abserr)
 };
 
 struct pasc_batch {
@@ -172,8 +180,18 @@ static int check_params(const pasc_spg_params *p) {
     return PASC_OK;
 }
 
+// A = alpha*A0: graphh1fem.cpp:146-151 (with FEM reduction the coarse problem keeps the
+// fine T in the scalef denominator, and without scalef scales by T2/T)
+static double problem_alpha(const pasc_problem *p) {
+    const double Tn = (double)p->T;
+    if (p->fem_type != PASC_FEM_NONE)
+        return p->scalef ? p->epssqr / Tn : p->epssqr * (double)p->fg.T2 / Tn;
+    return p->scalef ? p->epssqr / Tn : p->epssqr;
+}
+
 static void fill_desc(pasc_problem *p, InstDesc &d) {
-    const int64_t n = p->grid_width > 0 ? p->R : p->T;
+    const bool fem = p->fem_type != PASC_FEM_NONE;
+    const int64_t n = fem ? p->fg.T2 : (p->grid_width > 0 ? p->R : p->T);
     d.T = n;
     d.ld = (n + 63) / 64 * 64;
     d.goff = 0;
@@ -181,10 +199,11 @@ static void fill_desc(pasc_problem *p, InstDesc &d) {
     d.K = p->K;
     d.xdim = p->xdim;
     const double Tn = (double)p->T;  // the scalef denominator is the number of time steps
-    d.alpha = p->scalef ? p->epssqr / Tn : p->epssqr;
+    d.alpha = problem_alpha(p);
     d.bscale = p->scalef ? 1.0 / Tn : 1.0;
-    d.s_kkt = p->scalef ? Tn : 1.0;
-    d.heldout = p->d_heldout;
+    // KKT step: the time steps of the grid the QP lives on (the coarse one with FEM)
+    d.s_kkt = p->scalef ? (fem ? (double)p->fg.T2 : Tn) : 1.0;
+    d.heldout = fem ? nullptr : p->d_heldout;
 }
 
 int pasc_problem_create(pasc_ctx *ctx, const pasc_problem_desc *desc, pasc_problem **out) {
@@ -196,6 +215,16 @@ int pasc_problem_create(pasc_ctx *ctx, const pasc_problem_desc *desc, pasc_probl
     if (desc->precision != PASC_FP64 && desc->precision != PASC_FP32_BULK)
         return fail(PASC_ERR_ARG, "unknown precision %d", desc->precision);
     const bool grid = desc->grid_width > 0;
+    const bool fem = desc->fem_reduce > 0.0 && desc->fem_reduce < 1.0 &&
+                     desc->fem_type != PASC_FEM_NONE;
+    int64_t T2 = 0;
+    if (fem) {
+        if (grid || desc->R != 1 || !desc->x)
+            return fail(PASC_ERR_ARG, "FEM reduction: 1-D signals with data only");
+        T2 = (int64_t)std::ceil((double)desc->T * desc->fem_reduce);  // fem.cpp:198
+        int frc = fem_check_geom(desc->fem_type, desc->T, T2);
+        if (frc) return frc;
+    }
     if (grid) {
         if (desc->T != 1 || desc->R % desc->grid_width != 0)
             return fail(PASC_ERR_ARG, "grid graph needs T == 1 and R a multiple of grid_width");
@@ -216,6 +245,11 @@ int pasc_problem_create(pasc_ctx *ctx, const pasc_problem_desc *desc, pasc_probl
     p->scalef = desc->scalef;
     p->precision = desc->precision;
     p->grid_width = desc->grid_width;
+    if (fem) {
+        p->fem_type = desc->fem_type;
+        const int64_t ld1 = (desc->T + 63) / 64 * 64, ld2 = (T2 + 63) / 64 * 64;
+        p->fg = fem_geom(desc->fem_type, desc->T, T2, ld1, ld2);
+    }
     const int64_t n = grid ? p->R : p->T;
     p->has_data = desc->x != nullptr;
     if (p->has_data) p->h_x.assign(desc->x, desc->x + n * p->xdim);
@@ -235,7 +269,7 @@ int pasc_problem_create(pasc_ctx *ctx, const pasc_problem_desc *desc, pasc_probl
     std::vector<InstDesc> descs{d};
     rc = engine_alloc(p->e, &ctx->c, p->precision, grid ? STENCIL_GRID : STENCIL_TIME,
                       p->grid_width, descs, d.ld * p->xdim);
-    if (rc == PASC_OK) {
+    if (rc == PASC_OK && !fem) {
         // data rows: SoA, row c holds channel c
         std::vector<double> rows((size_t)d.ld * p->xdim, 0.0);
         for (int c = 0; c < p->xdim; c++)
@@ -243,9 +277,33 @@ int pasc_problem_create(pasc_ctx *ctx, const pasc_problem_desc *desc, pasc_probl
                 rows[(size_t)c * d.ld + i] = desc->x[i * p->xdim + c];
         rc = engine_upload_data(p->e, rows.data(), 0, (int64_t)rows.size());
     }
+    if (rc == PASC_OK && fem) {
+        // the fine grid: data rows, gamma rows and a coarse scratch; the engine itself runs
+        // on the explicit linear term reduced from the fine residual before every solve
+        const int64_t ld1 = p->fg.ld1;
+        std::vector<double> rows((size_t)ld1 * p->xdim, 0.0);
+        for (int c = 0; c < p->xdim; c++)
+            for (int64_t i = 0; i < n; i++) rows[(size_t)c * ld1 + i] = desc->x[i * p->xdim + c];
+        hipError_t he = hipMalloc(&p->d_x1, sizeof(double) * rows.size());
+        if (he == hipSuccess)
+            he = hipMemcpy(p->d_x1, rows.data(), sizeof(double) * rows.size(), hipMemcpyHostToDevice);
+        if (he == hipSuccess) he = hipMalloc(&p->d_g1, sizeof(double) * (size_t)p->K * ld1);
+        if (he == hipSuccess) he = hipMemset(p->d_g1, 0, sizeof(double) * (size_t)p->K * ld1);
+        if (he == hipSuccess) he = hipMalloc(&p->d_tmp, sizeof(double) * (size_t)p->K * p->fg.ld2);
+        if (he != hipSuccess) rc = fail(PASC_ERR_HIP, "FEM buffers: %s", hipGetErrorString(he));
+        std::memset(&p->fine, 0, sizeof(p->fine));
+        p->fine.T = p->T;
+        p->fine.ld = ld1;
+        p->fine.K = p->K;
+        p->fine.xdim = p->xdim;
+        p->fine.heldout = p->d_heldout;
+    }
     if (rc) {
         engine_free(p->e);
         if (p->d_heldout) (void)hipFree(p->d_heldout);
+        if (p->d_x1) (void)hipFree(p->d_x1);
+        if (p->d_g1) (void)hipFree(p->d_g1);
+        if (p->d_tmp) (void)hipFree(p->d_tmp);
         delete p;
         return rc;
     }
@@ -258,6 +316,9 @@ int pasc_problem_destroy(pasc_problem *p) {
     (void)hipSetDevice(p->ctx->c.device);
     engine_free(p->e);
     if (p->d_heldout) (void)hipFree(p->d_heldout);
+    if (p->d_x1) (void)hipFree(p->d_x1);
+    if (p->d_g1) (void)hipFree(p->d_g1);
+    if (p->d_tmp) (void)hipFree(p->d_tmp);
     delete p;
     return PASC_OK;
 }
@@ -268,6 +329,7 @@ int pasc_set_theta(pasc_problem *p, const double *theta) {
     for (int i = 0; i < p->K * p->xdim; i++) {
         if (!std::isfinite(theta[i])) return fail(PASC_ERR_ARG, "set_theta: theta[%d] is not finite", i);
         p->e.h_desc[0].theta[i] = theta[i];
+        p->fine.theta[i] = theta[i];
     }
     p->theta_set = true;
     int rc = use_device(p->ctx);
@@ -287,6 +349,17 @@ int pasc_set_gamma(pasc_problem *p, const double *gamma, int layout) {
     if (layout != PASC_LAYOUT_TMAJOR && layout != PASC_LAYOUT_KMAJOR) return fail(PASC_ERR_ARG, "bad layout");
     int rc = use_device(p->ctx);
     if (rc) return rc;
+    if (p->fem_type != PASC_FEM_NONE) {
+        // the fine gamma (TSData::gammavector); it is reduced to the coarse grid per solve
+        const int64_t T = p->T, ld1 = p->fg.ld1;
+        std::vector<double> rows((size_t)p->K * ld1, 0.0);
+        for (int k = 0; k < p->K; k++)
+            for (int64_t t = 0; t < T; t++)
+                rows[k * ld1 + t] = layout == PASC_LAYOUT_TMAJOR ? gamma[t * p->K + k] : gamma[k * T + t];
+        HIP_TRY(hipMemcpy(p->d_g1, rows.data(), sizeof(double) * rows.size(), hipMemcpyHostToDevice));
+        p->gamma_set = true;
+        return PASC_OK;
+    }
     rc = engine_set_gamma(p->e, 0, gamma, layout);
     if (rc == PASC_OK) p->gamma_set = true;
     return rc;
@@ -297,6 +370,10 @@ int pasc_set_gamma_uniform(pasc_problem *p) {
     if (!p) return fail(PASC_ERR_ARG, "set_gamma_uniform: NULL argument");
     int rc = use_device(p->ctx);
     if (rc) return rc;
+    if (p->fem_type != PASC_FEM_NONE) {
+        std::vector<double> g((size_t)p->T * p->K, 1.0 / p->K);
+        return pasc_set_gamma(p, g.data(), PASC_LAYOUT_TMAJOR);
+    }
     rc = engine_set_uniform(p->e, 0);
     if (rc == PASC_OK) p->gamma_set = true;
     return rc;
@@ -309,6 +386,19 @@ int pasc_get_gamma(pasc_problem *p, double *gamma, int layout) {
     if (!p->gamma_set) return fail(PASC_ERR_STATE, "get_gamma: gamma was never set");
     int rc = use_device(p->ctx);
     if (rc) return rc;
+    if (p->fem_type != PASC_FEM_NONE) {
+        const int64_t T = p->T, ld1 = p->fg.ld1;
+        std::vector<double> rows((size_t)p->K * ld1);
+        HIP_TRY(hipMemcpyAsync(rows.data(), p->d_g1, sizeof(double) * rows.size(),
+                               hipMemcpyDeviceToHost, (hipStream_t)p->ctx->c.stream));
+        HIP_TRY(hipStreamSynchronize((hipStream_t)p->ctx->c.stream));
+        for (int k = 0; k < p->K; k++)
+            for (int64_t t = 0; t < T; t++) {
+                if (layout == PASC_LAYOUT_TMAJOR) gamma[t * p->K + k] = rows[k * ld1 + t];
+                else gamma[k * T + t] = rows[k * ld1 + t];
+            }
+        return PASC_OK;
+    }
     return engine_get_gamma(p->e, 0, gamma, layout);
 }
 
@@ -318,9 +408,52 @@ int pasc_set_epssqr(pasc_problem *p, double epssqr) {
     if (!(epssqr >= 0) || !std::isfinite(epssqr)) return fail(PASC_ERR_ARG, "epssqr must be finite and >= 0");
     p->epssqr = epssqr;
     InstDesc &d = p->e.h_desc[0];
-    d.alpha = p->scalef ? epssqr / (double)p->T : epssqr;
+    d.alpha = problem_alpha(p);
     int rc = use_device(p->ctx);
     if (rc) return rc;
+    return engine_update_desc(p->e);
+}
+
+// FEM: the coarse linear term b2 = -reduce((x - theta_k)^2) / T2 (scalef) and the coarse start
+// reduce(gamma) (graphh1fem.cpp:261-286), before the coarse solve
+static int fem_before_solve(pasc_problem *p) {
+    void *s = p->ctx->c.stream;
+    double *b2;
+    int rc = engine_linear_term_rows(p->e, 0, &b2);
+    if (rc) return rc;
+    const double bs = p->scalef ? -1.0 / (double)p->fg.T2 : -1.0;
+    rc = fem_residual_rows(s, p->fg, p->K, p->d_x1, p->xdim, p->fine.theta, p->d_heldout, b2, bs);
+    if (rc) return rc;
+    rc = engine_commit_linear_term(p->e, 0);
+    if (rc) return rc;
+    rc = fem_reduce_rows(s, p->fg, p->K, p->d_g1, p->d_tmp, 1.0);
+    if (rc) return rc;
+    return engine_set_gamma_rows(p->e, 0, p->d_tmp);
+}
+
+// the gamma step of GraphH1FEMModel: with FEM, reduce -> coarse solve -> prolongate
+// (graphh1fem.cpp:236-301); otherwise the solve itself
+static int problem_gamma_step(pasc_problem *p, const pasc_spg_params &sp, pasc_spg_result *r) {
+    if (p->fem_type == PASC_FEM_NONE) return engine_solve(p->e, sp, r, nullptr, nullptr);
+    int rc = fem_before_solve(p);
+    if (rc) return rc;
+    rc = engine_solve(p->e, sp, r, nullptr, nullptr);
+    if (rc) return rc;
+    bool f32 = false;
+    const void *g2 = engine_current_gamma(p->e, 0, &f32);
+    return fem_prolongate_rows(p->ctx->c.stream, p->fg, p->K, g2, f32, p->d_g1);
+}
+
+// the theta step: from the fine gamma with FEM
+static int problem_theta_step(pasc_problem *p) {
+    if (p->fem_type == PASC_FEM_NONE) return engine_theta_update(p->e, 0, nullptr);
+    double th[PASC_MAX_K * 3];
+    int rc = rows_theta(p->e, p->d_g1, false, p->fine, p->d_x1, th);
+    if (rc) return rc;
+    for (int i = 0; i < p->K * p->xdim; i++) {
+        p->e.h_desc[0].theta[i] = th[i];
+        p->fine.theta[i] = th[i];
+    }
     return engine_update_desc(p->e);
 }
 
@@ -336,13 +469,12 @@ int pasc_gamma_solve(pasc_problem *p, const pasc_spg_params *params, pasc_spg_re
     rc = use_device(p->ctx);
     if (rc) return rc;
     if (!p->gamma_set) {
-        rc = engine_set_uniform(p->e, 0);
+        rc = pasc_set_gamma_uniform(p);
         if (rc) return rc;
-        p->gamma_set = true;
     }
     pasc_spg_result r;
     std::memset(&r, 0, sizeof(r));
-    rc = engine_solve(p->e, *params, &r, nullptr, nullptr);
+    rc = problem_gamma_step(p, *params, &r);
     if (res) *res = r;
     if (rc == PASC_OK) p->solved = true;
     return rc;
@@ -351,6 +483,8 @@ int pasc_gamma_solve(pasc_problem *p, const pasc_spg_params *params, pasc_spg_re
 int pasc_set_linear_term(pasc_problem *p, const double *b, int layout) {
     clear_error();
     if (!p) return fail(PASC_ERR_ARG, "set_linear_term: NULL problem");
+    if (p->fem_type != PASC_FEM_NONE)
+        return fail(PASC_ERR_STATE, "set_linear_term: the FEM-reduced problem forms its own b");
     if (layout != PASC_LAYOUT_TMAJOR && layout != PASC_LAYOUT_KMAJOR) return fail(PASC_ERR_ARG, "bad layout");
     if (b) {
         const int64_t n = (p->grid_width > 0 ? p->R : p->T) * (int64_t)p->K;
@@ -374,7 +508,7 @@ int pasc_theta_update(pasc_problem *p) {
     if (!p->has_data) return fail(PASC_ERR_STATE, "theta_update: the problem has no data");
     int rc = use_device(p->ctx);
     if (rc) return rc;
-    rc = engine_theta_update(p->e, 0, nullptr);
+    rc = problem_theta_step(p);
     if (rc == PASC_OK) p->theta_set = true;
     return rc;
 }
@@ -390,22 +524,21 @@ int pasc_ts_solve(pasc_problem *p, const pasc_ts_params *tp, const pasc_spg_para
     rc = use_device(p->ctx);
     if (rc) return rc;
     if (!p->gamma_set) {
-        rc = engine_set_uniform(p->e, 0);
+        rc = pasc_set_gamma_uniform(p);
         if (rc) return rc;
-        p->gamma_set = true;
     }
     // tssolver.h:542-650
     double L = std::numeric_limits<double>::max(), L_old, deltaL = L;
     int it, spg_sum = 0, increased = 0;
     for (it = 0; it < tp->maxit; it++) {
         if (!tp->theta_fixed && !(it == 0 && tp->skip_first_theta)) {
-            rc = engine_theta_update(p->e, 0, nullptr);
+            rc = problem_theta_step(p);
             if (rc) return rc;
             p->theta_set = true;
         }
         if (!p->theta_set) return fail(PASC_ERR_STATE, "ts_solve: theta was never set");
         pasc_spg_result r;
-        rc = engine_solve(p->e, *sp, &r, nullptr, nullptr);
+        rc = problem_gamma_step(p, *sp, &r);
         if (rc) return rc;
         spg_sum += r.it;
         L_old = L;
@@ -442,6 +575,7 @@ int pasc_abserr_reconstructed(pasc_problem *p, const double *solution, double *a
     if (!p->gamma_set) return fail(PASC_ERR_STATE, "abserr: gamma was never set");
     int rc = use_device(p->ctx);
     if (rc) return rc;
+    if (p->fem_type != PASC_FEM_NONE) return rows_abserr(p->e, p->d_g1, false, p->fine, solution, abserr);
     return engine_abserr(p->e, 0, solution, abserr);
 }
 
@@ -451,6 +585,7 @@ int pasc_get_labels(pasc_problem *p, int32_t *labels) {
     if (!p->gamma_set) return fail(PASC_ERR_STATE, "labels: gamma was never set");
     int rc = use_device(p->ctx);
     if (rc) return rc;
+    if (p->fem_type != PASC_FEM_NONE) return rows_labels(p->e, p->d_g1, false, p->fine, labels);
     return engine_labels(p->e, 0, labels);
 }
 
@@ -581,12 +716,54 @@ int pasc_adjoint(pasc_problem *p, const uint8_t *mask, double cg_tol, int32_t cg
     if (p->xdim != 1 || p->grid_width > 0) return fail(PASC_ERR_ARG, "adjoint: 1-D signals (xdim == 1) only");
     if (!p->solved) return fail(PASC_ERR_STATE, "adjoint: solve the gamma problem first");
     if (p->b_set || !p->has_data) return fail(PASC_ERR_STATE, "adjoint: needs the data term (no explicit b)");
+    if (p->fem_type != PASC_FEM_NONE) return fail(PASC_ERR_ARG, "adjoint: not for FEM-reduced problems");
     int rc = use_device(p->ctx);
     if (rc) return rc;
     int it = 0;
     rc = engine_adjoint(p->e, 0, mask, cg_tol, cg_maxit, loss, dlogeps, dtheta, &it);
     if (cg_it) *cg_it = it;
     return rc;
+}
+
+int64_t pasc_fem_reduced_T(int64_t T1, double fem_reduce) {
+    if (!(fem_reduce > 0.0) || fem_reduce >= 1.0) return T1;
+    return (int64_t)std::ceil((double)T1 * fem_reduce);
+}
+
+static int fem_kernel_args(pasc_ctx *ctx, const void *a, const void *b, int64_t T1, int64_t T2,
+                           int32_t K, int64_t ld1, int64_t ld2, int32_t type) {
+    if (!ctx || !a || !b) return fail(PASC_ERR_ARG, "fem: NULL argument");
+    if (K < 1 || K > PASC_MAX_K) return fail(PASC_ERR_ARG, "fem: K must be in [1, %d]", PASC_MAX_K);
+    if (ld1 < T1 || ld2 < T2) return fail(PASC_ERR_ARG, "fem: row stride shorter than the row");
+    return fem_check_geom(type, T1, T2);
+}
+
+int pasc_fem_reduce(pasc_ctx *ctx, const double *fine_dev, double *coarse_dev, int64_t T1,
+                    int64_t T2, int32_t K, int64_t ld1, int64_t ld2, int32_t fem_type) {
+    clear_error();
+    int rc = fem_kernel_args(ctx, fine_dev, coarse_dev, T1, T2, K, ld1, ld2, fem_type);
+    if (rc) return rc;
+    rc = use_device(ctx);
+    if (rc) return rc;
+    rc = fem_reduce_rows(ctx->c.stream, fem_geom(fem_type, T1, T2, ld1, ld2), K, fine_dev,
+                         coarse_dev, 1.0);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize((hipStream_t)ctx->c.stream));
+    return PASC_OK;
+}
+
+int pasc_fem_prolongate(pasc_ctx *ctx, const double *coarse_dev, double *fine_dev, int64_t T1,
+                        int64_t T2, int32_t K, int64_t ld1, int64_t ld2, int32_t fem_type) {
+    clear_error();
+    int rc = fem_kernel_args(ctx, coarse_dev, fine_dev, T1, T2, K, ld1, ld2, fem_type);
+    if (rc) return rc;
+    rc = use_device(ctx);
+    if (rc) return rc;
+    rc = fem_prolongate_rows(ctx->c.stream, fem_geom(fem_type, T1, T2, ld1, ld2), K, coarse_dev,
+                             false, fine_dev);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize((hipStream_t)ctx->c.stream));
+    return PASC_OK;
 }
 
 int pasc_device_copy_bandwidth(pasc_ctx *ctx, int64_t bytes, int32_t reps, double *gbps) {
@@ -619,6 +796,14 @@ int pasc_autotune(pasc_problem *p, const char *cache_path, pasc_launch_shape *sh
     if (!p->theta_set && !p->b_set) return fail(PASC_ERR_STATE, "autotune: neither theta nor b was set");
     int rc = use_device(p->ctx);
     if (rc) return rc;
+    if (p->fem_type != PASC_FEM_NONE) {
+        if (!p->gamma_set) {
+            rc = pasc_set_gamma_uniform(p);
+            if (rc) return rc;
+        }
+        rc = fem_before_solve(p);  // the coarse QP the autotuner times
+        if (rc) return rc;
+    }
     pasc_launch_shape sh{0, 0, 0, 0, 0.0};
     // the choice is cached per context, keyed by device, K, precision and log2(T)
     char key[128];

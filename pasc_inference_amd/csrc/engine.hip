@@ -218,6 +218,8 @@ static int scratch(Engine &e, size_t bytes, void **p) {
     return PASC_OK;
 }
 
+static int alloc_linear_term(Engine &e);
+
 int engine_upload_data(Engine &e, const double *x_host, int64_t offset, int64_t n) {
     HIP_TRY(hipMemcpyAsync((double *)e.d_x64 + offset, x_host, 8 * (size_t)n,
                            hipMemcpyHostToDevice, S_(e)));
@@ -300,14 +302,8 @@ int engine_set_linear_term(Engine &e, int inst, const double *b_host, int layout
         d.b32 = nullptr;
         return engine_update_desc(e);
     }
-    if (!e.d_b64) {
-        HIP_TRY(hipMalloc(&e.d_b64, 8 * (size_t)e.state_elems));
-        HIP_TRY(hipMemsetAsync(e.d_b64, 0, 8 * (size_t)e.state_elems, S_(e)));
-        if (e.precision == PASC_FP32_BULK) {
-            HIP_TRY(hipMalloc(&e.d_b32, 4 * (size_t)e.state_elems));
-            HIP_TRY(hipMemsetAsync(e.d_b32, 0, 4 * (size_t)e.state_elems, S_(e)));
-        }
-    }
+    int arc = alloc_linear_term(e);
+    if (arc) return arc;
     const int64_t n = (int64_t)d.K * d.ld;
     std::vector<double> rows((size_t)n, 0.0);
     for (int k = 0; k < d.K; k++)
@@ -749,10 +745,8 @@ int engine_eval(Engine &e, int inst, double *fx, double *kkt) {
     return engine_eval_raw(e, inst, fx, kkt, nullptr);
 }
 
-int engine_theta_update(Engine &e, int inst, double *theta_out) {
-    InstDesc &d = e.h_desc[inst];
-    bool f32;
-    const void *g = cur_gamma(e, inst, f32);
+int rows_theta(Engine &e, const void *g, bool f32, const InstDesc &d, const double *x,
+               double *theta_out) {
     const int blocks = (int)((d.T + 255) / 256);
     const int nv = d.K * (1 + d.xdim);
     void *part;
@@ -760,10 +754,10 @@ int engine_theta_update(Engine &e, int inst, double *theta_out) {
     if (rc) return rc;
     if (f32)
         hipLaunchKernelGGL((theta_partials<float>), dim3(blocks), dim3(256), 0, S_(e),
-                           (const float *)g + d.goff, (const double *)e.d_x64, d, (double *)part);
+                           (const float *)g, x, d, (double *)part);
     else
         hipLaunchKernelGGL((theta_partials<double>), dim3(blocks), dim3(256), 0, S_(e),
-                           (const double *)g + d.goff, (const double *)e.d_x64, d, (double *)part);
+                           (const double *)g, x, d, (double *)part);
     HIP_TRY(hipGetLastError());
     std::vector<double> h((size_t)nv * blocks);
     HIP_TRY(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, S_(e)));
@@ -775,37 +769,52 @@ int engine_theta_update(Engine &e, int inst, double *theta_out) {
     for (int k = 0; k < d.K; k++)
         for (int c = 0; c < d.xdim; c++) {
             const double gs = tot[k];
-            const double v = gs != 0 ? tot[d.K + k * d.xdim + c] / gs : 0.0;
-            d.theta[k * d.xdim + c] = v;
-            if (theta_out) theta_out[k * d.xdim + c] = v;
+            theta_out[k * d.xdim + c] = gs != 0 ? tot[d.K + k * d.xdim + c] / gs : 0.0;
         }
+    return PASC_OK;
+}
+
+int engine_theta_update(Engine &e, int inst, double *theta_out) {
+    InstDesc &d = e.h_desc[inst];
+    bool f32;
+    const void *g = cur_gamma(e, inst, f32);
+    double th[PASC_MAX_K * 3];
+    const size_t es = f32 ? 4 : 8;
+    int rc = rows_theta(e, (const char *)g + es * d.goff, f32, d, (const double *)e.d_x64, th);
+    if (rc) return rc;
+    for (int i = 0; i < d.K * d.xdim; i++) {
+        d.theta[i] = th[i];
+        if (theta_out) theta_out[i] = th[i];
+    }
     return engine_update_desc(e);
 }
 
-int engine_labels(Engine &e, int inst, int32_t *labels) {
-    const InstDesc &d = e.h_desc[inst];
-    bool f32;
-    const void *g = cur_gamma(e, inst, f32);
+int rows_labels(Engine &e, const void *g, bool f32, const InstDesc &d, int32_t *labels) {
     const int blocks = (int)((d.T + 255) / 256);
     void *buf;
     int rc = scratch(e, sizeof(int32_t) * (size_t)d.T, &buf);
     if (rc) return rc;
     if (f32)
         hipLaunchKernelGGL((labels_kernel<float>), dim3(blocks), dim3(256), 0, S_(e),
-                           (const float *)g + d.goff, d, (int32_t *)buf);
+                           (const float *)g, d, (int32_t *)buf);
     else
         hipLaunchKernelGGL((labels_kernel<double>), dim3(blocks), dim3(256), 0, S_(e),
-                           (const double *)g + d.goff, d, (int32_t *)buf);
+                           (const double *)g, d, (int32_t *)buf);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(labels, buf, sizeof(int32_t) * d.T, hipMemcpyDeviceToHost, S_(e)));
     HIP_TRY(hipStreamSynchronize(S_(e)));
     return PASC_OK;
 }
 
-int engine_abserr(Engine &e, int inst, const double *solution, double *abserr) {
+int engine_labels(Engine &e, int inst, int32_t *labels) {
     const InstDesc &d = e.h_desc[inst];
     bool f32;
     const void *g = cur_gamma(e, inst, f32);
+    return rows_labels(e, (const char *)g + (f32 ? 4 : 8) * d.goff, f32, d, labels);
+}
+
+int rows_abserr(Engine &e, const void *g, bool f32, const InstDesc &d, const double *solution,
+                double *abserr) {
     const int blocks = (int)((d.T + 255) / 256);
     void *buf;
     const size_t need = sizeof(double) * ((size_t)d.T + blocks);
@@ -816,10 +825,10 @@ int engine_abserr(Engine &e, int inst, const double *solution, double *abserr) {
     HIP_TRY(hipMemcpyAsync(sol, solution, sizeof(double) * d.T, hipMemcpyHostToDevice, S_(e)));
     if (f32)
         hipLaunchKernelGGL((abserr_kernel<float>), dim3(blocks), dim3(256), 0, S_(e),
-                           (const float *)g + d.goff, d, (const double *)sol, part);
+                           (const float *)g, d, (const double *)sol, part);
     else
         hipLaunchKernelGGL((abserr_kernel<double>), dim3(blocks), dim3(256), 0, S_(e),
-                           (const double *)g + d.goff, d, (const double *)sol, part);
+                           (const double *)g, d, (const double *)sol, part);
     HIP_TRY(hipGetLastError());
     std::vector<double> h(blocks);
     HIP_TRY(hipMemcpyAsync(h.data(), part, sizeof(double) * blocks, hipMemcpyDeviceToHost, S_(e)));
@@ -828,6 +837,70 @@ int engine_abserr(Engine &e, int inst, const double *solution, double *abserr) {
     for (double v : h) a += v;
     *abserr = a;
     return PASC_OK;
+}
+
+int engine_abserr(Engine &e, int inst, const double *solution, double *abserr) {
+    const InstDesc &d = e.h_desc[inst];
+    bool f32;
+    const void *g = cur_gamma(e, inst, f32);
+    return rows_abserr(e, (const char *)g + (f32 ? 4 : 8) * d.goff, f32, d, solution, abserr);
+}
+
+const void *engine_current_gamma(Engine &e, int inst, bool *f32) {
+    bool f;
+    const void *g = cur_gamma(e, inst, f);
+    if (f32) *f32 = f;
+    return (const char *)g + (f ? 4 : 8) * e.h_desc[inst].goff;
+}
+
+int engine_set_gamma_rows(Engine &e, int inst, const double *rows_dev) {
+    const InstDesc &d = e.h_desc[inst];
+    const int64_t n = (int64_t)d.K * d.ld;
+    e.cur64 = false;
+    e.h_parity[inst] = 0;
+    if (e.precision == PASC_FP32_BULK) {
+        hipLaunchKernelGGL((convert_kernel<double, float>), dim3((unsigned)((n + 255) / 256)),
+                           dim3(256), 0, S_(e), rows_dev, (float *)e.d_gamma[0] + d.goff, n);
+        HIP_TRY(hipGetLastError());
+    } else {
+        HIP_TRY(hipMemcpyAsync((double *)e.d_gamma[0] + d.goff, rows_dev, 8 * (size_t)n,
+                               hipMemcpyDeviceToDevice, S_(e)));
+    }
+    return PASC_OK;
+}
+
+static int alloc_linear_term(Engine &e) {
+    if (e.d_b64) return PASC_OK;
+    HIP_TRY(hipMalloc(&e.d_b64, 8 * (size_t)e.state_elems));
+    HIP_TRY(hipMemsetAsync(e.d_b64, 0, 8 * (size_t)e.state_elems, S_(e)));
+    if (e.precision == PASC_FP32_BULK) {
+        HIP_TRY(hipMalloc(&e.d_b32, 4 * (size_t)e.state_elems));
+        HIP_TRY(hipMemsetAsync(e.d_b32, 0, 4 * (size_t)e.state_elems, S_(e)));
+    }
+    return PASC_OK;
+}
+
+int engine_linear_term_rows(Engine &e, int inst, double **rows) {
+    int rc = alloc_linear_term(e);
+    if (rc) return rc;
+    *rows = (double *)e.d_b64 + e.h_desc[inst].goff;
+    return PASC_OK;
+}
+
+int engine_commit_linear_term(Engine &e, int inst) {
+    InstDesc &d = e.h_desc[inst];
+    const int64_t n = (int64_t)d.K * d.ld;
+    double *b64 = (double *)e.d_b64 + d.goff;
+    float *b32 = nullptr;
+    if (e.d_b32) {
+        b32 = (float *)e.d_b32 + d.goff;
+        hipLaunchKernelGGL((convert_kernel<double, float>), dim3((unsigned)((n + 255) / 256)),
+                           dim3(256), 0, S_(e), (const double *)b64, b32, n);
+        HIP_TRY(hipGetLastError());
+    }
+    d.b64 = b64;
+    d.b32 = b32;
+    return engine_update_desc(e);
 }
 
 template <typename S>

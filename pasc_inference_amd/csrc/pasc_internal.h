@@ -162,6 +162,36 @@ int engine_autotune(Engine &e, int *block, int *chunk, int *prefetch, int *max_s
 int engine_set_shape(Engine &e, int block, int chunk, int prefetch, int max_slots);
 
 int device_copy_bandwidth(Ctx *ctx, int64_t bytes, int reps, double *gbps);
+
+// device-side hand-over between the engine and code outside it (FEM coarsening)
+int engine_linear_term_rows(Engine &e, int inst, double **rows);  // FP64 rows to fill
+int engine_commit_linear_term(Engine &e, int inst);               // FP32 copy + descriptor
+int engine_set_gamma_rows(Engine &e, int inst, const double *rows_dev);
+const void *engine_current_gamma(Engine &e, int inst, bool *f32);
+// theta step, labels and abserr on gamma rows described by d (gamma g of the given type,
+// data rows x of stride d.ld); theta_out receives theta[k*xdim + c]
+int rows_theta(Engine &e, const void *g, bool f32, const InstDesc &d, const double *x,
+               double *theta_out);
+int rows_labels(Engine &e, const void *g, bool f32, const InstDesc &d, int32_t *labels);
+int rows_abserr(Engine &e, const void *g, bool f32, const InstDesc &d, const double *solution,
+                double *abserr);
+
+// fem.hip
+struct FemGeom {
+    int type;            // PASC_FEM_CONSTANT or PASC_FEM_HAT
+    int64_t T1, T2;      // fine and coarse steps
+    int64_t ld1, ld2;    // row strides
+    double diff;
+    int64_t right_idx;   // FemHat: fine steps at or beyond it leave the right half of a hat
+};
+FemGeom fem_geom(int type, int64_t T1, int64_t T2, int64_t ld1, int64_t ld2);
+int fem_check_geom(int type, int64_t T1, int64_t T2);
+int fem_reduce_rows(void *stream, const FemGeom &g, int K, const double *in, double *out,
+                    double scale);
+int fem_residual_rows(void *stream, const FemGeom &g, int K, const double *x, int xdim,
+                      const double *theta, const uint8_t *heldout, double *out, double scale);
+int fem_prolongate_rows(void *stream, const FemGeom &g, int K, const void *in, bool in_f32,
+                        double *out);
 int kernel_project(Ctx *ctx, void *x, int64_t nblocks, int K, int dtype, int layout, int64_t ld);
 int kernel_matvec(Ctx *ctx, const void *x, void *y, int64_t T, int64_t R, int K, int64_t ld,
                   double alpha, int32_t grid_width, int dtype);

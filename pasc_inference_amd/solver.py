@@ -47,7 +47,8 @@ class Context:
 class Problem:
     def __init__(self, ctx: Context, x, K: int, T: int | None = None, R: int = 1, xdim: int = 1,
                  scalef: bool = True, precision: int = capi.FP64, heldout=None,
-                 grid_width: int = 0, epssqr: float = 1.0):
+                 grid_width: int = 0, epssqr: float = 1.0, fem_reduce: float = 1.0,
+                 fem_type: int = capi.FEM_NONE):
         self.ctx = ctx
         if x is None:  # no data term: the QP is defined by an explicit b
             if T is None:
@@ -69,6 +70,7 @@ class Problem:
         d.x = dptr(self.x) if self.x is not None else None
         d.heldout = u8ptr(self.heldout) if self.heldout is not None else None
         d.grid_width = grid_width
+        d.fem_reduce, d.fem_type = float(fem_reduce), int(fem_type)
         h = C.c_void_p()
         check(lib().pasc_problem_create(ctx.h, C.byref(d), C.byref(h)))
         self.h = h

@@ -37,6 +37,11 @@ int main(int argc, char **argv) {
     opt.set_option_value("test_precision", &precision, std::string("fp64"));
     opt.set_option_value("test_skip_first_theta", &skip_first_theta, true);
     opt.set_option_value("test_device", &device, 0);
+    // test_signal1D.cpp:81-82: 0 = Fem (cell sums), 1 = FemHat; fem_reduce 1 = no reduction
+    int fem_type_ref;
+    double fem_reduce;
+    opt.set_option_value("test_fem_type", &fem_type_ref, 1);
+    opt.set_option_value("test_fem_reduce", &fem_reduce, 1.0);
     if (annealing != 1) {
         std::cout << "test_annealing: only 1 (no random restarts) is supported" << std::endl;
         return 1;
@@ -54,12 +59,13 @@ int main(int argc, char **argv) {
     int rc = 0;
     {
         const int prec = precision == "fp32" ? PASC_FP32_BULK : PASC_FP64;
-        data::Signal1DData<HipVector> mydata(ctx, filename, K, prec);
+        data::Signal1DData<HipVector> mydata(ctx, filename, K, prec, fem_reduce,
+                                             fem_type_ref == 0 ? PASC_FEM_CONSTANT : PASC_FEM_HAT);
         if (!mydata.ok()) return 2;
         std::vector<double> solution = petscio::read_vec(filename_solution);
         std::cout << "- PROBLEM INFO ----------------------------" << std::endl;
         std::cout << " T = " << mydata.get_T() << ", K = " << K << ", precision = " << precision
-                  << std::endl;
+                  << ", fem_reduce = " << fem_reduce << " (type " << fem_type_ref << ")" << std::endl;
 
         model::GraphH1FEMModel<HipVector> mymodel(mydata, epssqr_list[0], opt);
         solver::TSSolver<HipVector> mysolver(mydata, mymodel, opt);

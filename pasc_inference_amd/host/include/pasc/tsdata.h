@@ -12,9 +12,12 @@ template <class VectorBase> class TSData;
 
 template <> class TSData<HipVector> {
   public:
-    // T time steps of xdim values (R = 1), K clusters, precision PASC_FP64 / PASC_FP32_BULK
+    // T time steps of xdim values (R = 1), K clusters, precision PASC_FP64 / PASC_FP32_BULK.
+    // fem_reduce < 1 solves the gamma QP on a coarse grid (the reference passes a Fem / FemHat
+    // object to GraphH1FEMModel, test_signal1D.cpp:218-228; here the problem owns it).
     TSData(pasc_ctx *ctx, const std::vector<double> &x, int K, int xdim = 1,
-           int precision = PASC_FP64, const std::vector<uint8_t> *heldout = nullptr)
+           int precision = PASC_FP64, const std::vector<uint8_t> *heldout = nullptr,
+           double fem_reduce = 1.0, int fem_type = PASC_FEM_HAT)
         : ctx_(ctx), x_(x), K_(K), xdim_(xdim) {
         T_ = (int64_t)x.size() / xdim;
         pasc_problem_desc d;
@@ -26,6 +29,8 @@ template <> class TSData<HipVector> {
         d.scalef = 1;
         d.precision = precision;
         d.x = x_.data();
+        d.fem_reduce = fem_reduce;
+        d.fem_type = fem_type;
         if (heldout) {
             heldout_ = *heldout;
             d.heldout = heldout_.data();
@@ -95,10 +100,13 @@ template <class VectorBase> class Signal1DData;
 // 1-D signal read from a PETSc binary Vec (signal1Ddata.cpp:7-30)
 template <> class Signal1DData<HipVector> : public TSData<HipVector> {
   public:
-    Signal1DData(pasc_ctx *ctx, const std::string &filename, int K, int precision = PASC_FP64)
-        : TSData<HipVector>(ctx, petscio::read_vec(filename), K, 1, precision) {}
-    Signal1DData(pasc_ctx *ctx, const std::vector<double> &x, int K, int precision = PASC_FP64)
-        : TSData<HipVector>(ctx, x, K, 1, precision) {}
+    Signal1DData(pasc_ctx *ctx, const std::string &filename, int K, int precision = PASC_FP64,
+                 double fem_reduce = 1.0, int fem_type = PASC_FEM_HAT)
+        : TSData<HipVector>(ctx, petscio::read_vec(filename), K, 1, precision, nullptr,
+                            fem_reduce, fem_type) {}
+    Signal1DData(pasc_ctx *ctx, const std::vector<double> &x, int K, int precision = PASC_FP64,
+                 double fem_reduce = 1.0, int fem_type = PASC_FEM_HAT)
+        : TSData<HipVector>(ctx, x, K, 1, precision, nullptr, fem_reduce, fem_type) {}
 
     // compute_abserr_reconstructed (signal1Ddata.cpp:133-191): || sum_k theta_k gamma_k - sol ||_1
     double compute_abserr_reconstructed(const std::vector<double> &solution) const {

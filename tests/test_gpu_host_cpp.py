@@ -59,3 +59,40 @@ def test_cpp_driver_epsilon_scan_on_reference_fixture(oracle, tmp_path):
     assert np.mean(g.reshape(T, K).argmax(1) == gref.reshape(T, K).argmax(1)) >= 0.995
     lines = open(tmp_path / "short.csv").read().strip().splitlines()
     assert len(lines) == 5 and lines[0].startswith("K,epssqr,abserr,")
+
+
+def test_cpp_driver_fem_reduced_scan(oracle, tmp_path):
+    """test_fem_type / test_fem_reduce (test_signal1D.cpp:81-82,218-228): the epsilon scan with
+    the gamma QP solved on a 5x coarser grid, against the oracle's FEM alternation."""
+    from pasc_inference_amd import petsc_io
+    x = os.path.join(DATA, "signal1D_id1_idSigma1.bin")
+    sol = os.path.join(DATA, "samplesignal_solution.bin")
+    xs, truth = petsc_io.read_vec(x), petsc_io.read_vec(sol)
+    T, K, r = xs.size, 2, 0.2
+    eps_list = [1.0, 10.0]
+    for ref_type, ftype in [(0, oracle.FEM_CONSTANT), (1, oracle.FEM_HAT)]:
+        out = str(tmp_path / f"fem{ref_type}")
+        cmd = [DRIVER, f"--test_filename={x}", f"--test_filename_solution={sol}", "--test_K=2",
+               "--test_Theta=1", "--test_Theta=2", f"--test_filename_out={out}",
+               f"--test_shortinfo_filename={out}.csv", f"--test_fem_type={ref_type}",
+               f"--test_fem_reduce={r}", "--spgqpsolver_stop_difff=0", "--spgqpsolver_stop_kkt=1",
+               "--spgqpsolver_kkt_tol=1e-11", "--spgqpsolver_maxit=200000"]
+        cmd += [f"--test_epssqr={e}" for e in eps_list]
+        print(" ".join(cmd), flush=True)
+        rr = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+        print(rr.stdout[-1500:], rr.stderr[-1000:])
+        assert rr.returncode == 0
+        rows = [l.split(",") for l in open(out + ".csv").read().strip().splitlines()[1:]]
+        abserr_dev = [float(row[2]) for row in rows]
+        g = np.full(T * K, 1.0 / K)
+        th = np.array([1.0, 2.0])
+        for e, a_dev in zip(eps_list, abserr_dev):
+            g, _, _ = oracle.ts_solve(xs, g, th, T, 1, K, 1, e,
+                                      ts=dict(theta_fixed=1, fem_reduce=r, fem_type=ftype),
+                                      maxit=200000, stop_difff=0, stop_kkt=1, kkt_tol=1e-12,
+                                      kkt_scale=T * r, nthreads=8, gd_centered=1)
+            a_ref = oracle.abserr_reconstructed(g, th, truth, T, 1, K)
+            print(f"fem type {ref_type} eps {e}: abserr device {a_dev} oracle {a_ref}")
+            assert abs(a_dev - a_ref) <= 1e-4 * a_ref + 1e-3
+        gk = petsc_io.read_vec(out + "_gamma.bin")
+        assert gk.size == T * K  # the saved gamma is the fine (prolongated) one

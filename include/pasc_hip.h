@@ -74,11 +74,15 @@ typedef struct {
     int32_t precision;         /* pasc_precision */
     const double *x;           /* host data, [T][R][xdim]; NULL: no data term -- the QP is then
                                   defined by an explicit b (pasc_set_linear_term) only */
-    const int64_t *nbr_ptr;    /* optional CSR adjacency of the R nodes (R+1 entries) */
+    const int64_t *nbr_ptr;    /* optional CSR adjacency of the R nodes (R+1 entries; symmetric,
+                                  no self loops): the spatio-temporal graph of
+                                  blockgraphsparse.cpp:48-109 on T steps of R nodes */
     const int32_t *nbr_idx;    /* neighbour ids */
     const uint8_t *heldout;    /* optional [T*R] mask: held-out nodes carry no data term */
-    int32_t grid_width;        /* > 0: the graph is the 4-neighbour grid of width grid_width
-                                  (R = width*height); nbr_ptr/nbr_idx may then be NULL */
+    int32_t grid_width;        /* > 0: the R nodes form the 4-neighbour grid of width grid_width
+                                  (R = width*height; T = 1 for an image, T > 1 a video);
+                                  nbr_ptr/nbr_idx are then NULL.  Neither given and R > 1: R
+                                  unconnected nodes, each with its own time chain */
     double fem_reduce;         /* graphh1femmodel_fem_reduce: in (0, 1) the gamma QP is solved on
                                   a coarse time grid of T2 = ceil(T*fem_reduce) steps
                                   (graphh1fem.cpp:115-214,236-301); 0 or >= 1: no reduction */
@@ -214,10 +218,17 @@ int pasc_get_labels(pasc_problem *p, int32_t *labels);
 int pasc_project_simplex(pasc_ctx *ctx, void *x_dev, int64_t nblocks, int32_t K, int dtype,
                          int layout, int64_t ld);
 /* BlockGraphSparseMatrix<PetscVector>::matmult (blockgraphsparse.cpp:151-194): y = alpha*A0*x
- * for the 1-D time graph (R = 1) or the 4-neighbour grid graph (T = 1, R = width*height).
- * x, y are k-major device arrays with row stride ld. */
+ * for the spatio-temporal graph of T steps of R nodes (node n = t*R + r): the 1-D time graph
+ * (R = 1, grid_width = 0), R unconnected nodes (grid_width = 0), or the 4-neighbour grid of
+ * width grid_width (R = width*height), any T.  x, y are k-major device arrays, each row holding
+ * the T*R nodes with stride ld >= T*R. */
 int pasc_h1_matvec(pasc_ctx *ctx, const void *x_dev, void *y_dev, int64_t T, int64_t R,
                    int32_t K, int64_t ld, double alpha, int32_t grid_width, int dtype);
+/* The same for a general graph given as a CSR adjacency of the R nodes (host arrays,
+ * nbr_ptr[R+1], nbr_idx[nbr_ptr[R]]; blockgraphsparse.cpp:48-109 with BGraph's neighbours). */
+int pasc_h1_matvec_csr(pasc_ctx *ctx, const void *x_dev, void *y_dev, int64_t T, int64_t R,
+                       int32_t K, int64_t ld, double alpha, const int64_t *nbr_ptr,
+                       const int32_t *nbr_idx, int dtype);
 
 /* ---- FEM coarsening, kernel level (caller-owned device memory, k-major FP64 rows) ----
  * Fem<PetscVector>::reduce_gamma / prolongate_gamma (fem.cpp:28-166) for PASC_FEM_CONSTANT and

@@ -124,6 +124,8 @@ _SIGS = {
                              C.c_int),
     "pasc_h1_matvec": ([vp, vp, vp, C.c_int64, C.c_int64, C.c_int32, C.c_int64, C.c_double,
                         C.c_int32, C.c_int], C.c_int),
+    "pasc_h1_matvec_csr": ([vp, vp, vp, C.c_int64, C.c_int64, C.c_int32, C.c_int64, C.c_double,
+                            C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.c_int], C.c_int),
     "pasc_batch_create": ([vp, C.c_int32, C.POINTER(SeriesDesc), C.c_int32,
                            C.POINTER(C.c_double), C.c_int32, C.POINTER(vp)], C.c_int),
     "pasc_batch_destroy": ([vp], C.c_int),

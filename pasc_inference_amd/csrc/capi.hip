@@ -1,6 +1,7 @@
 // C ABI of libpasc_hip.so (declared in include/pasc_hip.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -60,6 +61,7 @@ struct pasc_problem {
     Engine e;
     // FEM coarsening (1-D): the engine holds the coarse QP of T2 steps, the fine grid of T
     // steps lives here (data, gamma) and meets the engine through reduce / prolongate
+    bool graph = false;        // node-wise graph stencil (R > 1, image grid or CSR adjacency)
     int fem_type = PASC_FEM_NONE;
     FemGeom fg{};
     double *d_x1 = nullptr;    // fine data rows [xdim][ld1]
@@ -191,7 +193,7 @@ static double problem_alpha(const pasc_problem *p) {
 
 static void fill_desc(pasc_problem *p, InstDesc &d) {
     const bool fem = p->fem_type != PASC_FEM_NONE;
-    const int64_t n = fem ? p->fg.T2 : (p->grid_width > 0 ? p->R : p->T);
+    const int64_t n = fem ? p->fg.T2 : p->T * p->R;  // nodes of the QP's graph
     d.T = n;
     d.ld = (n + 63) / 64 * 64;
     d.goff = 0;
@@ -225,13 +227,32 @@ int pasc_problem_create(pasc_ctx *ctx, const pasc_problem_desc *desc, pasc_probl
         int frc = fem_check_geom(desc->fem_type, desc->T, T2);
         if (frc) return frc;
     }
-    if (grid) {
-        if (desc->T != 1 || desc->R % desc->grid_width != 0)
-            return fail(PASC_ERR_ARG, "grid graph needs T == 1 and R a multiple of grid_width");
-    } else {
-        if (desc->R != 1 || desc->nbr_ptr || desc->nbr_idx)
-            return fail(PASC_ERR_ARG, "only the 1-D time graph (R == 1) and the 4-neighbour grid "
-                                      "(grid_width > 0, T == 1) are supported on the device");
+    const bool csr = desc->nbr_ptr != nullptr;
+    if (grid && csr) return fail(PASC_ERR_ARG, "give either grid_width or a CSR adjacency, not both");
+    if (grid && desc->R % desc->grid_width != 0)
+        return fail(PASC_ERR_ARG, "grid graph needs R a multiple of grid_width");
+    if (csr) {
+        // the reference's graphs are undirected: symmetric adjacency, no self loops
+        if (!desc->nbr_idx && desc->nbr_ptr[desc->R] > 0) return fail(PASC_ERR_ARG, "nbr_idx is NULL");
+        const int64_t *pt = desc->nbr_ptr;
+        const int32_t *ix = desc->nbr_idx;
+        if (pt[0] != 0) return fail(PASC_ERR_ARG, "nbr_ptr[0] must be 0");
+        std::vector<std::pair<int32_t, int32_t>> edges;
+        for (int64_t r = 0; r < desc->R; r++) {
+            if (pt[r + 1] < pt[r]) return fail(PASC_ERR_ARG, "nbr_ptr is not monotone");
+            for (int64_t e = pt[r]; e < pt[r + 1]; e++) {
+                if (ix[e] < 0 || ix[e] >= desc->R || ix[e] == r)
+                    return fail(PASC_ERR_ARG, "nbr_idx[%lld] out of range or a self loop", (long long)e);
+                edges.push_back({(int32_t)r, ix[e]});
+            }
+        }
+        std::vector<std::pair<int32_t, int32_t>> rev(edges.size());
+        for (size_t q = 0; q < edges.size(); q++) rev[q] = {edges[q].second, edges[q].first};
+        std::sort(edges.begin(), edges.end());
+        std::sort(rev.begin(), rev.end());
+        if (edges != rev) return fail(PASC_ERR_ARG, "the adjacency is not symmetric");
+        if (std::adjacent_find(edges.begin(), edges.end()) != edges.end())
+            return fail(PASC_ERR_ARG, "the adjacency lists a neighbour twice");
     }
     int rc = use_device(ctx);
     if (rc) return rc;
@@ -245,12 +266,13 @@ int pasc_problem_create(pasc_ctx *ctx, const pasc_problem_desc *desc, pasc_probl
     p->scalef = desc->scalef;
     p->precision = desc->precision;
     p->grid_width = desc->grid_width;
+    p->graph = grid || csr || desc->R > 1;
     if (fem) {
         p->fem_type = desc->fem_type;
         const int64_t ld1 = (desc->T + 63) / 64 * 64, ld2 = (T2 + 63) / 64 * 64;
         p->fg = fem_geom(desc->fem_type, desc->T, T2, ld1, ld2);
     }
-    const int64_t n = grid ? p->R : p->T;
+    const int64_t n = p->T * p->R;  // nodes
     p->has_data = desc->x != nullptr;
     if (p->has_data) p->h_x.assign(desc->x, desc->x + n * p->xdim);
     if (desc->heldout) {
@@ -267,8 +289,10 @@ int pasc_problem_create(pasc_ctx *ctx, const pasc_problem_desc *desc, pasc_probl
     std::memset(&d, 0, sizeof(d));
     fill_desc(p, d);
     std::vector<InstDesc> descs{d};
-    rc = engine_alloc(p->e, &ctx->c, p->precision, grid ? STENCIL_GRID : STENCIL_TIME,
+    rc = engine_alloc(p->e, &ctx->c, p->precision, p->graph ? STENCIL_GRID : STENCIL_TIME,
                       p->grid_width, descs, d.ld * p->xdim);
+    if (rc == PASC_OK && p->graph)
+        rc = engine_set_graph(p->e, p->T, p->R, p->grid_width, desc->nbr_ptr, desc->nbr_idx);
     if (rc == PASC_OK && !fem) {
         // data rows: SoA, row c holds channel c
         std::vector<double> rows((size_t)d.ld * p->xdim, 0.0);
@@ -487,7 +511,7 @@ int pasc_set_linear_term(pasc_problem *p, const double *b, int layout) {
         return fail(PASC_ERR_STATE, "set_linear_term: the FEM-reduced problem forms its own b");
     if (layout != PASC_LAYOUT_TMAJOR && layout != PASC_LAYOUT_KMAJOR) return fail(PASC_ERR_ARG, "bad layout");
     if (b) {
-        const int64_t n = (p->grid_width > 0 ? p->R : p->T) * (int64_t)p->K;
+        const int64_t n = p->T * p->R * (int64_t)p->K;
         for (int64_t i = 0; i < n; i++)
             if (!std::isfinite(b[i])) return fail(PASC_ERR_ARG, "set_linear_term: b[%lld] is not finite", (long long)i);
     }
@@ -607,17 +631,47 @@ int pasc_h1_matvec(pasc_ctx *ctx, const void *x_dev, void *y_dev, int64_t T, int
     clear_error();
     if (!ctx || !x_dev || !y_dev) return fail(PASC_ERR_ARG, "matvec: NULL argument");
     if (K < 1 || T < 1 || R < 1) return fail(PASC_ERR_ARG, "matvec: bad sizes");
-    if (grid_width > 0) {
-        if (T != 1 || R % grid_width) return fail(PASC_ERR_ARG, "matvec: grid needs T == 1, R % width == 0");
-        if (ld < R) return fail(PASC_ERR_ARG, "matvec: ld < R");
-    } else {
-        if (R != 1) return fail(PASC_ERR_ARG, "matvec: the time graph needs R == 1");
-        if (ld < T) return fail(PASC_ERR_ARG, "matvec: ld < T");
-    }
+    if (grid_width > 0 && R % grid_width) return fail(PASC_ERR_ARG, "matvec: grid needs R % width == 0");
+    if (ld < T * R) return fail(PASC_ERR_ARG, "matvec: ld < T*R");
     if (dtype != PASC_DTYPE_F32 && dtype != PASC_DTYPE_F64) return fail(PASC_ERR_ARG, "bad dtype");
     int rc = use_device(ctx);
     if (rc) return rc;
-    return kernel_matvec(&ctx->c, x_dev, y_dev, T, R, K, ld, alpha, grid_width, dtype);
+    return kernel_matvec(&ctx->c, x_dev, y_dev, T, R, K, ld, alpha, grid_width, nullptr, nullptr,
+                         dtype);
+}
+
+int pasc_h1_matvec_csr(pasc_ctx *ctx, const void *x_dev, void *y_dev, int64_t T, int64_t R,
+                       int32_t K, int64_t ld, double alpha, const int64_t *nbr_ptr,
+                       const int32_t *nbr_idx, int dtype) {
+    clear_error();
+    if (!ctx || !x_dev || !y_dev || !nbr_ptr) return fail(PASC_ERR_ARG, "matvec_csr: NULL argument");
+    if (K < 1 || T < 1 || R < 1) return fail(PASC_ERR_ARG, "matvec_csr: bad sizes");
+    if (ld < T * R) return fail(PASC_ERR_ARG, "matvec_csr: ld < T*R");
+    if (dtype != PASC_DTYPE_F32 && dtype != PASC_DTYPE_F64) return fail(PASC_ERR_ARG, "bad dtype");
+    const int64_t nnz = nbr_ptr[R];
+    if (nbr_ptr[0] != 0 || nnz < 0 || (nnz > 0 && !nbr_idx)) return fail(PASC_ERR_ARG, "matvec_csr: bad CSR");
+    for (int64_t r = 0; r < R; r++) {
+        if (nbr_ptr[r + 1] < nbr_ptr[r]) return fail(PASC_ERR_ARG, "matvec_csr: nbr_ptr is not monotone");
+        for (int64_t e = nbr_ptr[r]; e < nbr_ptr[r + 1]; e++)
+            if (nbr_idx[e] < 0 || nbr_idx[e] >= R) return fail(PASC_ERR_ARG, "matvec_csr: index out of range");
+    }
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    int64_t *dp = nullptr;
+    int32_t *di = nullptr;
+    HIP_TRY(hipMalloc(&dp, sizeof(int64_t) * (size_t)(R + 1)));
+    hipError_t he = hipMalloc(&di, sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1));
+    if (he == hipSuccess)
+        he = hipMemcpy(dp, nbr_ptr, sizeof(int64_t) * (size_t)(R + 1), hipMemcpyHostToDevice);
+    if (he == hipSuccess && nnz > 0)
+        he = hipMemcpy(di, nbr_idx, sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice);
+    if (he == hipSuccess)
+        rc = kernel_matvec(&ctx->c, x_dev, y_dev, T, R, K, ld, alpha, 0, dp, di, dtype);
+    else
+        rc = fail(PASC_ERR_HIP, "matvec_csr: %s", hipGetErrorString(he));
+    (void)hipFree(dp);
+    if (di) (void)hipFree(di);
+    return rc;
 }
 
 // ---- batch ----
@@ -713,7 +767,7 @@ int pasc_adjoint(pasc_problem *p, const uint8_t *mask, double cg_tol, int32_t cg
                  double *loss, double *dlogeps, double *dtheta, int32_t *cg_it) {
     clear_error();
     if (!p || !mask || !loss || !dlogeps || !dtheta) return fail(PASC_ERR_ARG, "adjoint: NULL argument");
-    if (p->xdim != 1 || p->grid_width > 0) return fail(PASC_ERR_ARG, "adjoint: 1-D signals (xdim == 1) only");
+    if (p->xdim != 1 || p->graph) return fail(PASC_ERR_ARG, "adjoint: 1-D signals (xdim == 1) only");
     if (!p->solved) return fail(PASC_ERR_STATE, "adjoint: solve the gamma problem first");
     if (p->b_set || !p->has_data) return fail(PASC_ERR_STATE, "adjoint: needs the data term (no explicit b)");
     if (p->fem_type != PASC_FEM_NONE) return fail(PASC_ERR_ARG, "adjoint: not for FEM-reduced problems");
@@ -783,7 +837,7 @@ int pasc_set_profiling(pasc_problem *p, int enable) {
 int pasc_set_launch_shape(pasc_problem *p, const pasc_launch_shape *shape) {
     clear_error();
     if (!p || !shape) return fail(PASC_ERR_ARG, "set_launch_shape: NULL argument");
-    if (p->grid_width > 0) return fail(PASC_ERR_ARG, "set_launch_shape: the grid path has a fixed shape");
+    if (p->graph) return fail(PASC_ERR_ARG, "set_launch_shape: the graph path has a fixed shape");
     int rc = use_device(p->ctx);
     if (rc) return rc;
     return engine_set_shape(p->e, shape->block, shape->chunk, shape->prefetch, shape->max_slots);
@@ -792,7 +846,7 @@ int pasc_set_launch_shape(pasc_problem *p, const pasc_launch_shape *shape) {
 int pasc_autotune(pasc_problem *p, const char *cache_path, pasc_launch_shape *shape) {
     clear_error();
     if (!p) return fail(PASC_ERR_ARG, "autotune: NULL problem");
-    if (p->grid_width > 0) return fail(PASC_ERR_ARG, "autotune: the grid path has a fixed shape");
+    if (p->graph) return fail(PASC_ERR_ARG, "autotune: the graph path has a fixed shape");
     if (!p->theta_set && !p->b_set) return fail(PASC_ERR_STATE, "autotune: neither theta nor b was set");
     int rc = use_device(p->ctx);
     if (rc) return rc;

@@ -195,6 +195,47 @@ __device__ inline void find_tile(const GroupView &g, int j, int &inst, int &chun
     chunk = j - g.off[a];
 }
 
+// Neighbour walks of the spatio-temporal graph (GraphGeo).  A0 is the weighted Laplacian with
+// edge weights 2 for (t,r)-(t+-1,r) and 1 for (t,r)-(t,s) and (t,r)-(t+-1,s), s a spatial
+// neighbour of r: its diagonal is Wsum = deg (T == 1), 2 deg + 2 (t at an end), 3 deg + 4
+// (interior t), as blockgraphsparse.cpp:57-73 assembles it.
+template <typename F>
+__device__ __forceinline__ void for_spatial(const GraphGeo &G, int64_t r, F &&f) {
+    if (G.ptr) {
+        for (int64_t e = G.ptr[r]; e < G.ptr[r + 1]; e++) f((int64_t)G.idx[e]);
+    } else if (G.width > 0) {
+        const int64_t W = G.width, row = r / W, col = r - row * W;
+        if (row > 0) f(r - W);
+        if (col > 0) f(r - 1);
+        if (col < W - 1) f(r + 1);
+        if (r + W < G.R) f(r + W);
+    }
+}
+
+// every neighbour n2 of node n with its edge weight w
+template <typename F>
+__device__ __forceinline__ void for_neighbours(const GraphGeo &G, int64_t n, F &&f) {
+    const int64_t t = n / G.R, r = n - t * G.R;
+    if (t > 0) f(n - G.R, 2.0);
+    if (t < G.T - 1) f(n + G.R, 2.0);
+    for_spatial(G, r, [&](int64_t s) {
+        f(t * G.R + s, 1.0);
+        if (t > 0) f((t - 1) * G.R + s, 1.0);
+        if (t < G.T - 1) f((t + 1) * G.R + s, 1.0);
+    });
+}
+
+// the edges of node n to nodes of larger index: every undirected edge once
+template <typename F>
+__device__ __forceinline__ void for_forward_edges(const GraphGeo &G, int64_t n, F &&f) {
+    const int64_t t = n / G.R, r = n - t * G.R;
+    if (t < G.T - 1) f(n + G.R, 2.0);
+    for_spatial(G, r, [&](int64_t s) {
+        if (s > r) f(t * G.R + s, 1.0);
+        if (t < G.T - 1) f((t + 1) * G.R + s, 1.0);
+    });
+}
+
 // Wave-uniform copies in scalar registers (the value of the first active lane).  Values that
 // every lane already holds equal become scalars, so later uses wait on nothing.
 __device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }

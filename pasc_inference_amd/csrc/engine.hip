@@ -185,6 +185,10 @@ void engine_free(Engine &e) {
     if (e.d_b64) (void)hipFree(e.d_b64);
     if (e.d_b32) (void)hipFree(e.d_b32);
     e.d_b64 = e.d_b32 = nullptr;
+    if (e.d_nbr_ptr) (void)hipFree(e.d_nbr_ptr);
+    if (e.d_nbr_idx) (void)hipFree(e.d_nbr_idx);
+    e.d_nbr_ptr = nullptr;
+    e.d_nbr_idx = nullptr;
     if (e.d_desc) (void)hipFree(e.d_desc);
     if (e.d_state) (void)hipFree(e.d_state);
     if (e.d_part) (void)hipFree(e.d_part);
@@ -219,6 +223,23 @@ static int scratch(Engine &e, size_t bytes, void **p) {
 }
 
 static int alloc_linear_term(Engine &e);
+
+int engine_set_graph(Engine &e, int64_t T, int64_t R, int32_t width, const int64_t *nbr_ptr,
+                     const int32_t *nbr_idx) {
+    e.geo = GraphGeo{T, R, width, nullptr, nullptr};
+    if (nbr_ptr) {
+        const int64_t nnz = nbr_ptr[R];
+        HIP_TRY(hipMalloc(&e.d_nbr_ptr, sizeof(int64_t) * (size_t)(R + 1)));
+        HIP_TRY(hipMalloc(&e.d_nbr_idx, sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)));
+        HIP_TRY(hipMemcpy(e.d_nbr_ptr, nbr_ptr, sizeof(int64_t) * (size_t)(R + 1), hipMemcpyHostToDevice));
+        if (nnz > 0)
+            HIP_TRY(hipMemcpy(e.d_nbr_idx, nbr_idx, sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice));
+        e.geo.ptr = e.d_nbr_ptr;
+        e.geo.idx = e.d_nbr_idx;
+        e.geo.width = 0;
+    }
+    return PASC_OK;
+}
 
 int engine_upload_data(Engine &e, const double *x_host, int64_t offset, int64_t n) {
     HIP_TRY(hipMemcpyAsync((double *)e.d_x64 + offset, x_host, 8 * (size_t)n,
@@ -370,7 +391,7 @@ static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<in
         } else {
             void *d0 = ph.f64 && e.precision == PASC_FP32_BULK ? e.d_dir64[0] : e.d_dir[0];
             void *d1 = ph.f64 && e.precision == PASC_FP32_BULK ? e.d_dir64[1] : e.d_dir[1];
-            GridArgs a{g0, g1, d0, d1, x, e.d_desc, e.d_state, e.d_part, gv, e.grid_width};
+            GridArgs a{g0, g1, d0, d1, x, e.d_desc, e.d_state, e.d_part, gv, e.geo};
             rc = ph.f64 ? launch_pass_grid_double(g.K, a, grids[gi], s)
                         : launch_pass_grid_float(g.K, a, grids[gi], s);
         }
@@ -695,14 +716,14 @@ template <int K>
 static void launch_eval(Engine &e, const InstDesc &d, const void *g, bool f32, double *part,
                         int blocks) {
     hipStream_t s = S_(e);
+    // the 1-D time graph of this instance, or the engine's graph
+    const GraphGeo G = e.stencil == STENCIL_TIME ? GraphGeo{d.T, 1, 0, nullptr, nullptr} : e.geo;
     if (f32)
         hipLaunchKernelGGL((eval_kernel<float, K>), dim3(blocks), dim3(256), 0, s,
-                           (const float *)g + d.goff, (const double *)e.d_x64, d, e.stencil,
-                           (int64_t)e.grid_width, part);
+                           (const float *)g + d.goff, (const double *)e.d_x64, d, G, part);
     else
         hipLaunchKernelGGL((eval_kernel<double, K>), dim3(blocks), dim3(256), 0, s,
-                           (const double *)g + d.goff, (const double *)e.d_x64, d, e.stencil,
-                           (int64_t)e.grid_width, part);
+                           (const double *)g + d.goff, (const double *)e.d_x64, d, G, part);
 }
 
 int engine_eval_raw(Engine &e, int inst, double *fx, double *kkt, double *nb2) {
@@ -1094,21 +1115,23 @@ int kernel_project(Ctx *ctx, void *x, int64_t nblocks, int K, int dtype, int lay
 }
 
 int kernel_matvec(Ctx *ctx, const void *x, void *y, int64_t T, int64_t R, int K, int64_t ld,
-                  double alpha, int32_t grid_width, int dtype) {
+                  double alpha, int32_t grid_width, const int64_t *nbr_ptr_dev,
+                  const int32_t *nbr_idx_dev, int dtype) {
     hipStream_t s = (hipStream_t)ctx->stream;
-    const int64_t n = grid_width > 0 ? R : T;
+    const int64_t n = T * R;
     if (n == 0) return PASC_OK;
     const dim3 grid((unsigned)((n + 255) / 256), (unsigned)K);
-    if (grid_width > 0) {
-        if (dtype == PASC_DTYPE_F32)
-            hipLaunchKernelGGL((matvec_grid<float>), grid, dim3(256), 0, s, (const float *)x, (float *)y, R, K, ld, (int64_t)grid_width, alpha);
-        else
-            hipLaunchKernelGGL((matvec_grid<double>), grid, dim3(256), 0, s, (const double *)x, (double *)y, R, K, ld, (int64_t)grid_width, alpha);
-    } else {
+    if (R == 1 && grid_width == 0 && !nbr_ptr_dev) {
         if (dtype == PASC_DTYPE_F32)
             hipLaunchKernelGGL((matvec_time<float>), grid, dim3(256), 0, s, (const float *)x, (float *)y, T, K, ld, alpha);
         else
             hipLaunchKernelGGL((matvec_time<double>), grid, dim3(256), 0, s, (const double *)x, (double *)y, T, K, ld, alpha);
+    } else {
+        const GraphGeo G{T, R, nbr_ptr_dev ? 0 : grid_width, nbr_ptr_dev, nbr_idx_dev};
+        if (dtype == PASC_DTYPE_F32)
+            hipLaunchKernelGGL((matvec_graph<float>), grid, dim3(256), 0, s, (const float *)x, (float *)y, G, K, ld, alpha);
+        else
+            hipLaunchKernelGGL((matvec_graph<double>), grid, dim3(256), 0, s, (const double *)x, (double *)y, G, K, ld, alpha);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));

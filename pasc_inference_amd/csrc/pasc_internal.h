@@ -15,6 +15,16 @@ void clear_error();
 
 constexpr int NPART = 5;  // per-tile partial sums: dd, dAd, gd, f, kkt
 
+// Spatio-temporal graph of blockgraphsparse.cpp:48-109: node n = t*R + r (t < T, r < R); the
+// spatial neighbours of r come from a symmetric CSR adjacency (ptr/idx, device arrays) or, with
+// ptr == nullptr and width > 0, from the 4-neighbour grid of that width (none otherwise).
+struct GraphGeo {
+    int64_t T, R;
+    int32_t width;
+    const int64_t *ptr;
+    const int32_t *idx;
+};
+
 // Read-only description of one gamma QP instance on the device.
 struct InstDesc {
     int64_t T;        // time steps (1-D) or nodes (grid graph, T == 1 in the reference sense)
@@ -72,6 +82,8 @@ struct StepParams {
     int32_t fx_identity;  // f_i = f_{i-1} + beta (gd + beta dAd / 2) instead of the FP32 sum
 };
 
+// STENCIL_TIME: 1-D time graph (R = 1, fused pass); STENCIL_GRID: any other graph of
+// blockgraphsparse.cpp (4-neighbour image grid or CSR adjacency, T >= 1), node-wise kernels
 enum StencilKind { STENCIL_TIME = 0, STENCIL_GRID = 1 };
 
 struct Ctx;
@@ -92,6 +104,9 @@ struct Engine {
     // device buffers
     void *d_gamma[2] = {nullptr, nullptr};    // ping-pong state buffers (precision type)
     void *d_gamma64[2] = {nullptr, nullptr};  // FP64 polish buffers (FP32-bulk only)
+    GraphGeo geo{};                           // graph stencil: T, R, grid width / CSR
+    int64_t *d_nbr_ptr = nullptr;             // CSR adjacency of the R nodes (device)
+    int32_t *d_nbr_idx = nullptr;
     void *d_dir[2] = {nullptr, nullptr};      // grid stencil: direction ping-pong
     void *d_dir64[2] = {nullptr, nullptr};
     void *d_x64 = nullptr;                    // data, FP64, rows of stride ld
@@ -143,6 +158,9 @@ int engine_alloc(Engine &e, Ctx *ctx, int precision, int stencil, int32_t grid_w
                  const std::vector<InstDesc> &descs, int64_t data_elems);
 void engine_free(Engine &e);
 int engine_upload_data(Engine &e, const double *x_host, int64_t offset, int64_t n);
+// graph stencil geometry: T time steps of R nodes, grid width or CSR adjacency (host arrays)
+int engine_set_graph(Engine &e, int64_t T, int64_t R, int32_t width, const int64_t *nbr_ptr,
+                     const int32_t *nbr_idx);
 int engine_set_linear_term(Engine &e, int inst, const double *b_host, int layout);
 int engine_set_uniform(Engine &e, int inst);
 int engine_fill_uniform(Engine &e);
@@ -194,6 +212,7 @@ int fem_prolongate_rows(void *stream, const FemGeom &g, int K, const void *in, b
                         double *out);
 int kernel_project(Ctx *ctx, void *x, int64_t nblocks, int K, int dtype, int layout, int64_t ld);
 int kernel_matvec(Ctx *ctx, const void *x, void *y, int64_t T, int64_t R, int K, int64_t ld,
-                  double alpha, int32_t grid_width, int dtype);
+                  double alpha, int32_t grid_width, const int64_t *nbr_ptr_dev,
+                  const int32_t *nbr_idx_dev, int dtype);
 
 }  // namespace pasc

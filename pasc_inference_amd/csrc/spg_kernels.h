@@ -479,10 +479,11 @@ __global__ __launch_bounds__(NT) void spg_pass_1d(PassArgs A) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Grid graph (T = 1, R = width*height, 4 neighbours; blockgraphsparse.cpp:51-66,93-104 with
-// T == 1: diagonal deg(r), -1 per neighbour).  Unfused: the pass rebuilds gamma_i pointwise
-// from gamma_{i-1} + beta*d_{i-1} (also at the 4 neighbours), forms g_i and d_i, and stores
-// gamma_i and d_i; a second kernel forms A*d_i.  Data is stored SoA, rows c = 0..xdim-1.
+// General graph stencil (image grids with T = 1, spatio-temporal graphs with T > 1 and R > 1,
+// CSR or implicit 4-neighbour grid; blockgraphsparse.cpp:48-109).  Unfused: the first kernel
+// rebuilds gamma_i node-wise from gamma_{i-1} + beta*d_{i-1} (also at the neighbours), forms
+// g_i and d_i, and stores gamma_i and d_i; the second forms |d|^2 and d'Ad over the edges.
+// Data is stored SoA, rows c = 0..xdim-1 of T*R nodes.
 // ---------------------------------------------------------------------------------------------
 struct GridArgs {
     void *gamma0;
@@ -494,7 +495,7 @@ struct GridArgs {
     const InstState *state;
     double *part;
     GroupView grp;
-    int32_t width;
+    GraphGeo geo;
 };
 
 template <typename S, typename C, int K>
@@ -549,33 +550,27 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_dir(GridArgs A) {
         find_tile(A.grp, j, inst, slot);
         const InstDesc &D = A.desc[inst];
         const InstState &st = A.state[inst];
-        const int64_t R = D.T;
-        const int64_t W = A.width;
+        const int64_t N = D.T;  // nodes T*R
         const int par = st.parity;
         double acc[NPART] = {0.0, 0.0, 0.0, 0.0, 0.0};
       for (int m = 0; m < D.msub; m++) {
         const int64_t r = (int64_t)(slot * D.msub + m) * NT + threadIdx.x;
-        if ((int64_t)(slot * D.msub + m) * NT >= R) break;
+        if ((int64_t)(slot * D.msub + m) * NT >= N) break;
         const S *src = reinterpret_cast<const S *>(par ? A.gamma1 : A.gamma0) + D.goff;
         S *dst = reinterpret_cast<S *>(par ? A.gamma0 : A.gamma1) + D.goff;
         S *dr = reinterpret_cast<S *>(par ? A.dir0 : A.dir1) + D.goff;
-        if (r < R) {
-            const int64_t row = r / W, col = r - row * W;
+        if (r < N) {
             C g0[K];
             grid_gamma_at<S, C, K>(A, D, st, src, r, g0);
             C lap[K];
 #pragma unroll
             for (int k = 0; k < K; k++) lap[k] = C(0);
-            const int64_t nb[4] = {row > 0 ? r - W : -1, col > 0 ? r - 1 : -1,
-                                   col < W - 1 ? r + 1 : -1, r + W < R ? r + W : -1};
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                if (nb[e] < 0) continue;
+            for_neighbours(A.geo, r, [&](int64_t n2, double w) {
                 C gn[K];
-                grid_gamma_at<S, C, K>(A, D, st, src, nb[e], gn);
+                grid_gamma_at<S, C, K>(A, D, st, src, n2, gn);
 #pragma unroll
-                for (int k = 0; k < K; k++) lap[k] += g0[k] - gn[k];
-            }
+                for (int k = 0; k < K; k++) lap[k] += C(w) * (g0[k] - gn[k]);
+            });
             C q[K];
             grid_dataterm<C, K>(D, A.x, r, is_f32, q);
             C y[K], p[K], z[K], g[K];
@@ -623,34 +618,24 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_ad(GridArgs A) {
         int inst, slot;
         find_tile(A.grp, j, inst, slot);
         const InstDesc &D = A.desc[inst];
-        const int64_t R = D.T;
-        const int64_t W = A.width;
+        const int64_t N = D.T;
         // d_i was written by spg_pass_grid_dir into the buffer opposite to the parity
         const S *dr = reinterpret_cast<const S *>(A.state[inst].parity ? A.dir0 : A.dir1) + D.goff;
         double acc[NPART] = {0.0, 0.0, 0.0, 0.0, 0.0};
       for (int m = 0; m < D.msub; m++) {
         const int64_t r = (int64_t)(slot * D.msub + m) * NT + threadIdx.x;
-        if ((int64_t)(slot * D.msub + m) * NT >= R) break;
-        if (r < R) {
-            const int64_t row = r / W, col = r - row * W;
-            const int64_t nb[4] = {row > 0 ? r - W : -1, col > 0 ? r - 1 : -1,
-                                   col < W - 1 ? r + 1 : -1, r + W < R ? r + W : -1};
-            (void)nb;
+        if ((int64_t)(slot * D.msub + m) * NT >= N) break;
+        if (r < N) {
             const double alpha = D.alpha;
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const double d = (double)dr[k * D.ld + r];
                 acc[0] += d * d;
-                // d'A d = alpha * sum over grid edges of (d_r - d_s)^2; each edge is counted
-                // once, from its left / upper end
-                if (col < W - 1) {
-                    const double e = d - (double)dr[k * D.ld + r + 1];
-                    acc[1] += alpha * e * e;
-                }
-                if (r + W < R) {
-                    const double e = d - (double)dr[k * D.ld + r + W];
-                    acc[1] += alpha * e * e;
-                }
+                // d'A d = alpha * sum over the graph's edges of w (d_a - d_b)^2, each edge once
+                for_forward_edges(A.geo, r, [&](int64_t n2, double w) {
+                    const double e = d - (double)dr[k * D.ld + n2];
+                    acc[1] += alpha * w * e * e;
+                });
             }
         }
       }

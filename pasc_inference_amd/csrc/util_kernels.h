@@ -93,28 +93,23 @@ __global__ __launch_bounds__(256) void matvec_time(const S *x, S *y, int64_t T, 
 }
 
 template <typename S>
-__global__ __launch_bounds__(256) void matvec_grid(const S *x, S *y, int64_t R, int K,
-                                                   int64_t ld, int64_t W, double alpha) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void matvec_graph(const S *x, S *y, GraphGeo G, int K,
+                                                    int64_t ld, double alpha) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
-    if (r >= R || k >= K) return;
+    if (n >= G.T * G.R || k >= K) return;
     const S *v = x + k * ld;
-    const int64_t row = r / W, col = r - row * W;
-    const double c = (double)v[r];
+    const double c = (double)v[n];
     double a = 0.0;
-    if (row > 0) a += c - (double)v[r - W];
-    if (col > 0) a += c - (double)v[r - 1];
-    if (col < W - 1) a += c - (double)v[r + 1];
-    if (r + W < R) a += c - (double)v[r + W];
-    y[k * ld + r] = S(alpha * a);
+    for_neighbours(G, n, [&](int64_t n2, double w) { a += w * (c - (double)v[n2]); });
+    y[k * ld + n] = S(alpha * a);
 }
 
 // ---- explicit FP64 evaluation of f = 1/2 g'Ag - b'g, the KKT residual and ||b||^2 -----------
 // Partials per block: [0] f, [1] ||b||^2, [2] kkt (max).
 template <typename S, int K>
 __global__ __launch_bounds__(256) void eval_kernel(const S *gam, const double *x,
-                                                   InstDesc D, int stencil, int64_t width,
-                                                   double *part) {
+                                                   InstDesc D, GraphGeo G, double *part) {
     __shared__ double lds[3 * 4];
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t T = D.T;
@@ -127,22 +122,10 @@ __global__ __launch_bounds__(256) void eval_kernel(const S *gam, const double *x
             lap[k] = 0.0;
             q[k] = 0.0;
         }
-        if (stencil == STENCIL_TIME) {
+        for_neighbours(G, t, [&](int64_t n2, double w) {
 #pragma unroll
-            for (int k = 0; k < K; k++) {
-                if (t > 0) lap[k] += 2.0 * (g0[k] - (double)gam[k * D.ld + t - 1]);
-                if (t < T - 1) lap[k] += 2.0 * (g0[k] - (double)gam[k * D.ld + t + 1]);
-            }
-        } else {
-            const int64_t row = t / width, col = t - row * width;
-            const int64_t nb[4] = {row > 0 ? t - width : -1, col > 0 ? t - 1 : -1,
-                                   col < width - 1 ? t + 1 : -1, t + width < T ? t + width : -1};
-            for (int e = 0; e < 4; e++) {
-                if (nb[e] < 0) continue;
-#pragma unroll
-                for (int k = 0; k < K; k++) lap[k] += g0[k] - (double)gam[k * D.ld + nb[e]];
-            }
-        }
+            for (int k = 0; k < K; k++) lap[k] += w * (g0[k] - (double)gam[k * D.ld + n2]);
+        });
         if (D.b64) {
 #pragma unroll
             for (int k = 0; k < K; k++) q[k] = -D.b64[k * D.ld + t];

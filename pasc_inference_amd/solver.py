@@ -48,8 +48,16 @@ class Problem:
     def __init__(self, ctx: Context, x, K: int, T: int | None = None, R: int = 1, xdim: int = 1,
                  scalef: bool = True, precision: int = capi.FP64, heldout=None,
                  grid_width: int = 0, epssqr: float = 1.0, fem_reduce: float = 1.0,
-                 fem_type: int = capi.FEM_NONE):
+                 fem_type: int = capi.FEM_NONE, graph=None):
+        """x: data [T][R][xdim] (None: no data term, T and R given).  The graph of the R nodes
+        is the 4-neighbour grid of width grid_width (T defaults to 1, an image), a CSR adjacency
+        graph = (nbr_ptr[R+1], nbr_idx), or none (R unconnected time chains)."""
         self.ctx = ctx
+        self._graph = None
+        if graph is not None:
+            self._graph = (np.ascontiguousarray(graph[0], dtype=np.int64),
+                           np.ascontiguousarray(graph[1], dtype=np.int32))
+            R = self._graph[0].size - 1
         if x is None:  # no data term: the QP is defined by an explicit b
             if T is None:
                 raise ValueError("x=None needs T (and R / grid_width)")
@@ -58,9 +66,13 @@ class Problem:
         else:
             self.x = np.ascontiguousarray(x, dtype=np.float64).ravel()
             n = self.x.size // xdim
-        if T is None:
-            T = n if grid_width == 0 else 1
-        self.T, self.R, self.K, self.xdim = T, R if grid_width == 0 else n, K, xdim
+            if grid_width > 0 and graph is None:
+                T = 1 if T is None else T
+                R = n // T
+            elif T is None:
+                T = n // R
+        assert T * R == n, "data size must be T*R*xdim"
+        self.T, self.R, self.K, self.xdim = T, R, K, xdim
         self.nodes = n
         self.grid_width = grid_width
         self.heldout = None if heldout is None else np.ascontiguousarray(heldout, dtype=np.uint8)
@@ -71,6 +83,9 @@ class Problem:
         d.heldout = u8ptr(self.heldout) if self.heldout is not None else None
         d.grid_width = grid_width
         d.fem_reduce, d.fem_type = float(fem_reduce), int(fem_type)
+        if self._graph is not None:
+            d.nbr_ptr = self._graph[0].ctypes.data_as(C.POINTER(C.c_int64))
+            d.nbr_idx = self._graph[1].ctypes.data_as(C.POINTER(C.c_int32))
         h = C.c_void_p()
         check(lib().pasc_problem_create(ctx.h, C.byref(d), C.byref(h)))
         self.h = h

@@ -24,3 +24,20 @@ def label_agreement(g1, g2, K):
     a = np.asarray(g1).reshape(-1, K).argmax(axis=1)
     b = np.asarray(g2).reshape(-1, K).argmax(axis=1)
     return float(np.mean(a == b))
+
+
+def random_graph_csr(R: int, mean_degree: float, seed: int):
+    """A random undirected graph on R nodes (symmetric adjacency, no self loops) as CSR."""
+    rng = np.random.default_rng(seed)
+    m = int(R * mean_degree / 2)
+    a = rng.integers(0, R, size=m)
+    b = rng.integers(0, R, size=m)
+    edges = {(int(min(i, j)), int(max(i, j))) for i, j in zip(a, b) if i != j}
+    nbr = [[] for _ in range(R)]
+    for i, j in sorted(edges):
+        nbr[i].append(j)
+        nbr[j].append(i)
+    ptr = np.zeros(R + 1, dtype=np.int64)
+    ptr[1:] = np.cumsum([len(v) for v in nbr])
+    idx = np.array([j for v in nbr for j in sorted(v)], dtype=np.int32)
+    return ptr, idx

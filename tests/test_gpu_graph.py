@@ -1,0 +1,121 @@
+"""The general graph stencil on the device against the oracle: spatio-temporal graphs of T
+steps of R nodes (blockgraphsparse.cpp:48-109; neighbours at t and t+-1), given as a CSR
+adjacency or as a 4-neighbour grid (videos: T > 1), and R unconnected time chains."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from devmem import DeviceArray
+from helpers import grid_csr, label_agreement, random_graph_csr
+
+pytestmark = pytest.mark.gpu
+
+
+def _rows(a, n, K, ld):
+    h = np.zeros((K, ld))
+    h[:, :n] = a.reshape(n, K).T
+    return h
+
+
+@pytest.mark.parametrize("T,R,kind", [(1, 50, "csr"), (7, 50, "csr"), (5, 12 * 9, "grid"),
+                                      (4, 33, "none"), (3, 1, "csr0")])
+def test_graph_matvec_matches_oracle(ctx, oracle, T, R, kind):
+    from pasc_inference_amd import capi
+    K = 3
+    rng = np.random.default_rng(T * 100 + R)
+    n = T * R
+    x = rng.random(n * K)
+    alpha = 0.61
+    ld = (n + 63) // 64 * 64
+    dx = DeviceArray(_rows(x, n, K, ld).ravel())
+    dy = DeviceArray(np.zeros(K * ld))
+    print(f"graph matvec T={T} R={R} {kind}", flush=True)
+    if kind in ("csr", "csr0"):
+        ptr, idx = random_graph_csr(R, 4.0, 3) if kind == "csr" else \
+            (np.zeros(R + 1, dtype=np.int64), np.zeros(0, dtype=np.int32))
+        capi.check(capi.lib().pasc_h1_matvec_csr(
+            ctx.h, dx.ptr, dy.ptr, T, R, K, ld, alpha,
+            ptr.ctypes.data_as(C.POINTER(C.c_int64)), idx.ctypes.data_as(C.POINTER(C.c_int32)),
+            capi.DTYPE_F64))
+        graph = (ptr, idx)
+    elif kind == "grid":
+        capi.check(capi.lib().pasc_h1_matvec(ctx.h, dx.ptr, dy.ptr, T, R, K, ld, alpha, 12,
+                                             capi.DTYPE_F64))
+        graph = grid_csr(12, 9)
+    else:
+        capi.check(capi.lib().pasc_h1_matvec(ctx.h, dx.ptr, dy.ptr, T, R, K, ld, alpha, 0,
+                                             capi.DTYPE_F64))
+        graph = (np.zeros(R + 1, dtype=np.int64), np.zeros(0, dtype=np.int32))
+    ref = oracle.h1_matvec(x, T, R, K, alpha, graph=graph)
+    out = dy.download().reshape(K, ld)[:, :n].T.ravel()
+    assert np.max(np.abs(out - ref)) < 1e-13
+
+
+def _video(seed, w, h, T, K, sigma):
+    """Rectangles that move one pixel per frame: T frames of a w x h image, one channel."""
+    rng = np.random.default_rng(seed)
+    lab = np.zeros((T, h, w), dtype=np.int64)
+    for k in range(1, K):
+        y0, x0 = rng.integers(0, h // 2), rng.integers(0, w // 2)
+        for t in range(T):
+            lab[t, y0:y0 + h // 3, (x0 + t) % w:(x0 + t) % w + w // 3] = k
+    levels = np.linspace(0.0, 2.0, K)
+    x = levels[lab] + rng.normal(0, sigma, size=lab.shape)
+    return x.ravel(), lab.ravel(), levels
+
+
+@pytest.mark.parametrize("prec_name", ["FP64", "FP32_BULK"])
+@pytest.mark.parametrize("kind", ["grid", "csr", "none"])
+def test_spatiotemporal_solve_matches_oracle(ctx, oracle, prec_name, kind):
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Problem
+    prec = getattr(capi, prec_name)
+    K, eps = 3, 2.0
+    if kind == "grid":
+        w, h, T = 16, 12, 6
+        R = w * h
+        x, lab, levels = _video(4, w, h, T, K, 0.4)
+        graph = grid_csr(w, h)
+        P = Problem(ctx, x, K, T=T, grid_width=w, precision=prec, epssqr=eps)
+    else:
+        R, T = 60, 8
+        graph = random_graph_csr(R, 3.0, 9) if kind == "csr" else \
+            (np.zeros(R + 1, dtype=np.int64), np.zeros(0, dtype=np.int32))
+        rng = np.random.default_rng(5)
+        lab = rng.integers(0, K, size=R)[None, :].repeat(T, 0).ravel()
+        levels = np.array([0.0, 1.0, 2.0])
+        x = levels[lab] + rng.normal(0, 0.4, size=T * R)
+        P = Problem(ctx, x, K, T=T, graph=graph if kind == "csr" else None, R=R, precision=prec,
+                    epssqr=eps)
+    theta = levels.copy()
+    b = oracle.assemble_b(x, theta, T, R, K)
+    ref, rr = oracle.spg_solve(np.full(T * R * K, 1.0 / K), b, T, R, K, eps / T, graph=graph,
+                               maxit=400000, stop_difff=0, stop_kkt=1, kkt_tol=1e-12, kkt_scale=T,
+                               nthreads=8, gd_centered=1)
+    assert rr.kkt <= 1e-12
+    P.set_theta(theta)
+    P.set_gamma_uniform()
+    r = P.gamma_solve(maxit=400000, stop_mask=capi.STOP_KKT, kkt_tol=1e-11)
+    g = P.get_gamma()
+    f, kkt = P.objective()
+    print(f"{kind} T={T} R={R} prec={prec_name}: it={r.it} kkt={kkt:.2e} f={f!r} oracle {rr.fx!r} "
+          f"err={np.max(np.abs(g - ref)):.2e}", flush=True)
+    assert r.stop_reason == capi.REASON_KKT and kkt <= 1e-11 * 1.001
+    assert np.max(np.abs(g - ref)) <= 1e-6
+    assert abs(f - rr.fx) <= 1e-8 * abs(rr.fx)
+    assert label_agreement(g, ref, K) >= 0.999
+    # theta step on the nodes (weighted means)
+    np.testing.assert_allclose(P.theta_update(), oracle.theta_update(x, ref, T, R, K), atol=1e-6)
+
+
+def test_graph_validation(ctx):
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Problem
+    x = np.zeros(4 * 3)
+    bad = [(np.array([0, 1, 1, 1], dtype=np.int64), np.array([1], dtype=np.int32)),   # not symmetric
+           (np.array([0, 1, 2, 2], dtype=np.int64), np.array([0, 0], dtype=np.int32)),  # self loop
+           (np.array([0, 1, 2, 3], dtype=np.int64), np.array([1, 0, 7], dtype=np.int32))]  # range
+    for g in bad:
+        with pytest.raises(capi.PascError):
+            Problem(ctx, x, 2, T=4, graph=g)

@@ -270,12 +270,19 @@ int pasc_batch_solve(pasc_batch *b, const pasc_spg_params *params, pasc_instance
                      double *time_s);
 int pasc_batch_get_gamma(pasc_batch *b, int32_t series, int32_t eps_index, double *gamma,
                          int layout);
+/* One problem (any graph, xdim, held-out mask; no FEM reduction) solved for n_eps values of
+ * epsilon at once: the same batched engine with a single series (series index 0). */
+int pasc_batch_create_problem(pasc_ctx *ctx, const pasc_problem_desc *desc,
+                              const double *theta, int32_t n_eps, const double *eps,
+                              pasc_batch **out);
 
 /* ---- adjoint (new capability) ----
  * Gradient of the held-out reconstruction loss  J = sum_{t in M} (x_t - sum_k theta_k gamma_kt)^2
  * with respect to log(epssqr) and theta, through the converged gamma of the last
  * pasc_gamma_solve (implicit differentiation on the active face of the simplex, projected CG).
- * mask is [T*R] (1 = held out).  dtheta has K entries (xdim = 1). */
+ * Any graph of the problem; with xdim > 1 the loss sums over the channels,
+ * J = sum_{n in M} sum_c (x_nc - sum_k theta_kc gamma_kn)^2.  mask is [T*R] (1 = held out).
+ * dtheta has K*xdim entries (theta layout). */
 int pasc_adjoint(pasc_problem *p, const uint8_t *mask, double cg_tol, int32_t cg_maxit,
                  double *loss, double *dlogeps, double *dtheta, int32_t *cg_it);
 

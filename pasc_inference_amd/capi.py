@@ -129,6 +129,8 @@ _SIGS = {
     "pasc_batch_create": ([vp, C.c_int32, C.POINTER(SeriesDesc), C.c_int32,
                            C.POINTER(C.c_double), C.c_int32, C.POINTER(vp)], C.c_int),
     "pasc_batch_destroy": ([vp], C.c_int),
+    "pasc_batch_create_problem": ([vp, C.POINTER(ProblemDesc), C.POINTER(C.c_double), C.c_int32,
+                                   C.POINTER(C.c_double), C.POINTER(vp)], C.c_int),
     "pasc_batch_solve": ([vp, C.POINTER(SpgParams), C.POINTER(InstanceResult),
                           C.POINTER(C.c_double)], C.c_int),
     "pasc_batch_get_gamma": ([vp, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.c_int],

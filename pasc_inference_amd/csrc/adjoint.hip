@@ -1,4 +1,4 @@
-// Adjoint of the converged gamma step (1-D time graph, xdim = 1).
+// Adjoint of the converged gamma step (any graph of the engine, xdim <= 3).
 //
 // At the minimiser gamma* of  min 1/2 g'Ag - b'g  over the product of simplices, let F be the
 // free set {(k,t): gamma_kt > 0} and P the orthogonal projector onto
@@ -30,15 +30,26 @@ constexpr double FREE_EPS = 1e-12;  // gamma entries at or below this are on the
 
 struct AdjArgs {
     const double *gam;      // gamma* (k-major, stride ld)
-    const double *x;        // data (stride-1 row)
-    const uint8_t *mask;    // held-out steps (1 = in the loss)
-    const uint8_t *qmask;   // steps without a data term in the QP (problem's held-out mask)
+    const double *x;        // data rows [xdim][ld]
+    const uint8_t *mask;    // held-out nodes (1 = in the loss)
+    const uint8_t *qmask;   // nodes without a data term in the QP (problem's held-out mask)
     uint8_t *freeset;       // [K][ld] 1 = free
-    int64_t T, ld;
-    int K;
+    int64_t T, ld;          // T: nodes
+    int K, xdim;
     double alpha, bscale;
-    double theta[PASC_MAX_K];
+    double theta[PASC_MAX_K * 3];
+    GraphGeo G;
 };
+
+constexpr int NTH = PASC_MAX_K * 3;  // theta entries
+
+// (A0 v)_n of one row
+__device__ inline double adj_lap(const AdjArgs &a, const double *row, int64_t n) {
+    const double c = row[n];
+    double l = 0.0;
+    for_neighbours(a.G, n, [&](int64_t n2, double w) { l += w * (c - row[n2]); });
+    return l;
+}
 
 template <int NV>
 __device__ inline void block_sum(double (&v)[NV], double *lds, double *out) {
@@ -69,36 +80,39 @@ __device__ inline void project_face(double *v, const bool *fr, int K) {
     for (int k = 0; k < K; k++) v[k] = (fr[k] && nf > 1) ? v[k] - m : 0.0;
 }
 
-// r = P dJ/dgamma; partials: [0] J, [1] r'r, [2 + k] direct dJ/dtheta_k
+// r = P dJ/dgamma; partials: [0] J, [1] r'r, [2 + k*xdim + c] direct dJ/dtheta_kc
 __global__ __launch_bounds__(ANT) void adj_init(AdjArgs a, double *r, double *part) {
-    __shared__ double lds[(2 + PASC_MAX_K) * (ANT / WAVE)];
+    __shared__ double lds[(2 + NTH) * (ANT / WAVE)];
     const int64_t t = (int64_t)blockIdx.x * ANT + threadIdx.x;
-    double v[2 + PASC_MAX_K];
-    for (int i = 0; i < 2 + PASC_MAX_K; i++) v[i] = 0.0;
+    double v[2 + NTH];
+    for (int i = 0; i < 2 + NTH; i++) v[i] = 0.0;
     if (t < a.T) {
         double g[PASC_MAX_K], dj[PASC_MAX_K];
         bool fr[PASC_MAX_K];
-        double rec = 0.0;
         for (int k = 0; k < a.K; k++) {
             g[k] = a.gam[k * a.ld + t];
             fr[k] = g[k] > FREE_EPS;
             a.freeset[k * a.ld + t] = fr[k] ? 1 : 0;
-            rec += a.theta[k] * g[k];
+            dj[k] = 0.0;
         }
         const bool held = a.mask[t] != 0;
-        const double res = held ? a.x[t] - rec : 0.0;
-        for (int k = 0; k < a.K; k++) {
-            dj[k] = -2.0 * res * a.theta[k];
-            v[2 + k] = -2.0 * res * g[k];
+        for (int c = 0; c < a.xdim; c++) {
+            double rec = 0.0;
+            for (int k = 0; k < a.K; k++) rec += a.theta[k * a.xdim + c] * g[k];
+            const double res = held ? a.x[c * a.ld + t] - rec : 0.0;
+            v[0] += res * res;
+            for (int k = 0; k < a.K; k++) {
+                dj[k] += -2.0 * res * a.theta[k * a.xdim + c];
+                v[2 + k * a.xdim + c] = -2.0 * res * g[k];
+            }
         }
-        v[0] = res * res;
         project_face(dj, fr, a.K);
         for (int k = 0; k < a.K; k++) {
             r[k * a.ld + t] = dj[k];
             v[1] += dj[k] * dj[k];
         }
     }
-    block_sum<2 + PASC_MAX_K>(v, lds, part);
+    block_sum<2 + NTH>(v, lds, part);
 }
 
 // q = P A P p (p already in range(P)); partial p'q
@@ -111,11 +125,7 @@ __global__ __launch_bounds__(ANT) void adj_matvec(AdjArgs a, const double *p, do
         double w[PASC_MAX_K];
         bool fr[PASC_MAX_K];
         for (int k = 0; k < a.K; k++) {
-            const double *row = p + k * a.ld;
-            double l = 0.0;
-            if (t > 0) l += row[t] - row[t - 1];
-            if (t < a.T - 1) l += row[t] - row[t + 1];
-            w[k] = a.alpha * 2.0 * l;
+            w[k] = a.alpha * adj_lap(a, p + k * a.ld, t);
             fr[k] = a.freeset[k * a.ld + t] != 0;
         }
         project_face(w, fr, a.K);
@@ -155,25 +165,24 @@ __global__ __launch_bounds__(ANT) void adj_pupdate(AdjArgs a, double b, const do
     }
 }
 
-// partials: [0] lambda' A gamma, [1 + k] sum_{t not held} lambda_kt * 2 (x_t - theta_k) * bscale
+// partials: [0] lambda' A gamma, [1 + k*xdim + c] sum_{n with data} lambda_kn 2 (x_nc - theta_kc) bscale
 __global__ __launch_bounds__(ANT) void adj_final(AdjArgs a, const double *lam, double *part) {
-    __shared__ double lds[(1 + PASC_MAX_K) * (ANT / WAVE)];
+    __shared__ double lds[(1 + NTH) * (ANT / WAVE)];
     const int64_t t = (int64_t)blockIdx.x * ANT + threadIdx.x;
-    double v[1 + PASC_MAX_K];
-    for (int i = 0; i < 1 + PASC_MAX_K; i++) v[i] = 0.0;
+    double v[1 + NTH];
+    for (int i = 0; i < 1 + NTH; i++) v[i] = 0.0;
     if (t < a.T) {
-        const bool nodata = a.qmask && a.qmask[t] != 0;  // b_kt does not depend on theta there
+        const bool nodata = a.qmask && a.qmask[t] != 0;  // b_kn does not depend on theta there
         for (int k = 0; k < a.K; k++) {
-            const double *row = a.gam + k * a.ld;
-            double l = 0.0;
-            if (t > 0) l += row[t] - row[t - 1];
-            if (t < a.T - 1) l += row[t] - row[t + 1];
             const double lk = lam[k * a.ld + t];
-            v[0] += lk * a.alpha * 2.0 * l;
-            if (!nodata) v[1 + k] = lk * 2.0 * (a.x[t] - a.theta[k]) * a.bscale;
+            v[0] += lk * a.alpha * adj_lap(a, a.gam + k * a.ld, t);
+            if (!nodata)
+                for (int c = 0; c < a.xdim; c++)
+                    v[1 + k * a.xdim + c] =
+                        lk * 2.0 * (a.x[c * a.ld + t] - a.theta[k * a.xdim + c]) * a.bscale;
         }
     }
-    block_sum<1 + PASC_MAX_K>(v, lds, part);
+    block_sum<1 + NTH>(v, lds, part);
 }
 
 static int sum_partials(hipStream_t s, const double *d_part, int blocks, int nv,
@@ -195,9 +204,8 @@ __global__ __launch_bounds__(256) void adj_convert(const A *src, B *dst, int64_t
 
 int engine_adjoint(Engine &e, int inst, const uint8_t *mask_host, double cg_tol, int cg_maxit,
                    double *loss, double *dlogeps, double *dtheta, int *cg_it) {
-    if (e.stencil != STENCIL_TIME) return fail(PASC_ERR_ARG, "adjoint: 1-D time graph only");
     const InstDesc &D = e.h_desc[inst];
-    if (D.xdim != 1) return fail(PASC_ERR_ARG, "adjoint: xdim must be 1");
+    if (D.b64) return fail(PASC_ERR_ARG, "adjoint: needs the data term");
     hipStream_t s = (hipStream_t)e.ctx->stream;
     const int64_t T = D.T, ld = D.ld, n = (int64_t)D.K * ld;
     const int blocks = (int)((T + ANT - 1) / ANT);
@@ -209,7 +217,7 @@ int engine_adjoint(Engine &e, int inst, const uint8_t *mask_host, double cg_tol,
     HIP_TRY(hipMalloc(&buf, sizeof(double) * 5 * (size_t)n));
     HIP_TRY(hipMalloc(&fr, (size_t)n));
     HIP_TRY(hipMalloc(&mk, (size_t)T));
-    HIP_TRY(hipMalloc(&part, sizeof(double) * (size_t)blocks * (2 + PASC_MAX_K)));
+    HIP_TRY(hipMalloc(&part, sizeof(double) * (size_t)blocks * (2 + NTH)));
     double *gam = buf, *lam = buf + n, *r = buf + 2 * n, *p = buf + 3 * n, *q = buf + 4 * n;
     int rc = PASC_OK;
     auto cleanup = [&]() {
@@ -240,18 +248,20 @@ int engine_adjoint(Engine &e, int inst, const uint8_t *mask_host, double cg_tol,
     a.T = T;
     a.ld = ld;
     a.K = D.K;
+    a.xdim = D.xdim;
     a.alpha = D.alpha;
     a.bscale = D.bscale;
-    for (int k = 0; k < PASC_MAX_K; k++) a.theta[k] = k < D.K ? D.theta[k] : 0.0;
+    for (int i = 0; i < NTH; i++) a.theta[i] = i < D.K * D.xdim ? D.theta[i] : 0.0;
+    a.G = e.stencil == STENCIL_TIME ? GraphGeo{T, 1, 0, nullptr, nullptr} : e.geo;
 
     std::vector<double> v;
     hipLaunchKernelGGL(adj_init, dim3(blocks), dim3(ANT), 0, s, a, r, part);
     HIP_TRY(hipGetLastError());
-    if ((rc = sum_partials(s, part, blocks, 2 + PASC_MAX_K, v))) { cleanup(); return rc; }
+    if ((rc = sum_partials(s, part, blocks, 2 + NTH, v))) { cleanup(); return rc; }
     *loss = v[0];
     double rr = v[1];
     const double rr0 = rr;
-    std::vector<double> direct(v.begin() + 2, v.begin() + 2 + D.K);
+    std::vector<double> direct(v.begin() + 2, v.begin() + 2 + D.K * D.xdim);
     HIP_TRY(hipMemcpyAsync(p, r, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
     int it = 0;
     // conjugate gradients on P A P (symmetric positive semi-definite on range(P))
@@ -270,9 +280,9 @@ int engine_adjoint(Engine &e, int inst, const uint8_t *mask_host, double cg_tol,
     }
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(adj_final, dim3(blocks), dim3(ANT), 0, s, a, lam, part);
-    if ((rc = sum_partials(s, part, blocks, 1 + PASC_MAX_K, v))) { cleanup(); return rc; }
+    if ((rc = sum_partials(s, part, blocks, 1 + NTH, v))) { cleanup(); return rc; }
     *dlogeps = -v[0];
-    for (int k = 0; k < D.K; k++) dtheta[k] = direct[k] + v[1 + k];
+    for (int i = 0; i < D.K * D.xdim; i++) dtheta[i] = direct[i] + v[1 + i];
     if (cg_it) *cg_it = it;
     cleanup();
     if (!std::isfinite(*dlogeps)) return fail(PASC_ERR_NUMERIC, "adjoint: non-finite gradient");

@@ -73,6 +73,7 @@ struct pasc_problem {
 struct pasc_batch {
     pasc_ctx *ctx = nullptr;
     int n_series = 0, n_eps = 0;
+    uint8_t *d_heldout = nullptr;  // shared data-term mask (single-problem batches)
     std::vector<int64_t> T;
     std::vector<int32_t> K;
     Engine e;
@@ -208,25 +209,14 @@ static void fill_desc(pasc_problem *p, InstDesc &d) {
     d.heldout = fem ? nullptr : p->d_heldout;
 }
 
-int pasc_problem_create(pasc_ctx *ctx, const pasc_problem_desc *desc, pasc_problem **out) {
-    clear_error();
-    if (!ctx || !desc || !out) return fail(PASC_ERR_ARG, "problem_create: NULL argument");
+// Sizes, precision and graph of a problem description (shared by problems and batches).
+static int validate_desc(const pasc_problem_desc *desc) {
     if (desc->T < 1 || desc->R < 1) return fail(PASC_ERR_ARG, "T and R must be >= 1");
     if (desc->K < 1 || desc->K > PASC_MAX_K) return fail(PASC_ERR_ARG, "K must be in [1, %d]", PASC_MAX_K);
     if (desc->xdim < 1 || desc->xdim > 3) return fail(PASC_ERR_ARG, "xdim must be in [1, 3]");
     if (desc->precision != PASC_FP64 && desc->precision != PASC_FP32_BULK)
         return fail(PASC_ERR_ARG, "unknown precision %d", desc->precision);
     const bool grid = desc->grid_width > 0;
-    const bool fem = desc->fem_reduce > 0.0 && desc->fem_reduce < 1.0 &&
-                     desc->fem_type != PASC_FEM_NONE;
-    int64_t T2 = 0;
-    if (fem) {
-        if (grid || desc->R != 1 || !desc->x)
-            return fail(PASC_ERR_ARG, "FEM reduction: 1-D signals with data only");
-        T2 = (int64_t)std::ceil((double)desc->T * desc->fem_reduce);  // fem.cpp:198
-        int frc = fem_check_geom(desc->fem_type, desc->T, T2);
-        if (frc) return frc;
-    }
     const bool csr = desc->nbr_ptr != nullptr;
     if (grid && csr) return fail(PASC_ERR_ARG, "give either grid_width or a CSR adjacency, not both");
     if (grid && desc->R % desc->grid_width != 0)
@@ -253,6 +243,26 @@ int pasc_problem_create(pasc_ctx *ctx, const pasc_problem_desc *desc, pasc_probl
         if (edges != rev) return fail(PASC_ERR_ARG, "the adjacency is not symmetric");
         if (std::adjacent_find(edges.begin(), edges.end()) != edges.end())
             return fail(PASC_ERR_ARG, "the adjacency lists a neighbour twice");
+    }
+    return PASC_OK;
+}
+
+int pasc_problem_create(pasc_ctx *ctx, const pasc_problem_desc *desc, pasc_problem **out) {
+    clear_error();
+    if (!ctx || !desc || !out) return fail(PASC_ERR_ARG, "problem_create: NULL argument");
+    int vrc = validate_desc(desc);
+    if (vrc) return vrc;
+    const bool grid = desc->grid_width > 0;
+    const bool csr = desc->nbr_ptr != nullptr;
+    const bool fem = desc->fem_reduce > 0.0 && desc->fem_reduce < 1.0 &&
+                     desc->fem_type != PASC_FEM_NONE;
+    int64_t T2 = 0;
+    if (fem) {
+        if (grid || desc->R != 1 || !desc->x)
+            return fail(PASC_ERR_ARG, "FEM reduction: 1-D signals with data only");
+        T2 = (int64_t)std::ceil((double)desc->T * desc->fem_reduce);  // fem.cpp:198
+        int frc = fem_check_geom(desc->fem_type, desc->T, T2);
+        if (frc) return frc;
     }
     int rc = use_device(ctx);
     if (rc) return rc;
@@ -731,10 +741,82 @@ int pasc_batch_create(pasc_ctx *ctx, int32_t n_series, const pasc_series_desc *s
     return PASC_OK;
 }
 
+int pasc_batch_create_problem(pasc_ctx *ctx, const pasc_problem_desc *desc,
+                              const double *theta, int32_t n_eps, const double *eps,
+                              pasc_batch **out) {
+    clear_error();
+    if (!ctx || !desc || !theta || !eps || !out || n_eps < 1)
+        return fail(PASC_ERR_ARG, "batch_create_problem: bad arguments");
+    int rc = validate_desc(desc);
+    if (rc) return rc;
+    if (!desc->x) return fail(PASC_ERR_ARG, "batch_create_problem: the batch needs data");
+    if (desc->fem_reduce > 0.0 && desc->fem_reduce < 1.0 && desc->fem_type != PASC_FEM_NONE)
+        return fail(PASC_ERR_ARG, "batch_create_problem: no FEM reduction in batches");
+    const bool graph = desc->grid_width > 0 || desc->nbr_ptr || desc->R > 1;
+    const int64_t n = desc->T * desc->R, ld = (n + 63) / 64 * 64;
+    const double Tn = (double)desc->T;
+    std::vector<InstDesc> descs;
+    for (int j = 0; j < n_eps; j++) {
+        if (!(eps[j] >= 0) || !std::isfinite(eps[j])) return fail(PASC_ERR_ARG, "bad eps[%d]", j);
+        InstDesc d;
+        std::memset(&d, 0, sizeof(d));
+        d.T = n;
+        d.ld = ld;
+        d.goff = (int64_t)j * desc->K * ld;
+        d.xoff = 0;  // every epsilon reads the same data rows
+        d.K = desc->K;
+        d.xdim = desc->xdim;
+        d.alpha = desc->scalef ? eps[j] / Tn : eps[j];
+        d.bscale = desc->scalef ? 1.0 / Tn : 1.0;
+        d.s_kkt = desc->scalef ? Tn : 1.0;
+        for (int i = 0; i < desc->K * desc->xdim; i++) {
+            if (!std::isfinite(theta[i])) return fail(PASC_ERR_ARG, "theta[%d] is not finite", i);
+            d.theta[i] = theta[i];
+        }
+        descs.push_back(d);
+    }
+    rc = use_device(ctx);
+    if (rc) return rc;
+    pasc_batch *b = new (std::nothrow) pasc_batch();
+    if (!b) return fail(PASC_ERR_ALLOC, "batch_create_problem: out of host memory");
+    b->ctx = ctx;
+    b->n_series = 1;
+    b->n_eps = n_eps;
+    if (desc->heldout) {
+        hipError_t he = hipMalloc(&b->d_heldout, (size_t)n);
+        if (he == hipSuccess) he = hipMemcpy(b->d_heldout, desc->heldout, (size_t)n, hipMemcpyHostToDevice);
+        if (he != hipSuccess) {
+            if (b->d_heldout) (void)hipFree(b->d_heldout);
+            delete b;
+            return fail(PASC_ERR_HIP, "heldout upload: %s", hipGetErrorString(he));
+        }
+        for (auto &d : descs) d.heldout = b->d_heldout;
+    }
+    rc = engine_alloc(b->e, &ctx->c, desc->precision, graph ? STENCIL_GRID : STENCIL_TIME,
+                      desc->grid_width, descs, ld * desc->xdim);
+    if (rc == PASC_OK && graph)
+        rc = engine_set_graph(b->e, desc->T, desc->R, desc->grid_width, desc->nbr_ptr, desc->nbr_idx);
+    if (rc == PASC_OK) {
+        std::vector<double> rows((size_t)ld * desc->xdim, 0.0);
+        for (int c = 0; c < desc->xdim; c++)
+            for (int64_t i = 0; i < n; i++) rows[(size_t)c * ld + i] = desc->x[i * desc->xdim + c];
+        rc = engine_upload_data(b->e, rows.data(), 0, (int64_t)rows.size());
+    }
+    if (rc) {
+        engine_free(b->e);
+        if (b->d_heldout) (void)hipFree(b->d_heldout);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return PASC_OK;
+}
+
 int pasc_batch_destroy(pasc_batch *b) {
     if (!b) return PASC_OK;
     (void)hipSetDevice(b->ctx->c.device);
     engine_free(b->e);
+    if (b->d_heldout) (void)hipFree(b->d_heldout);
     delete b;
     return PASC_OK;
 }
@@ -767,7 +849,6 @@ int pasc_adjoint(pasc_problem *p, const uint8_t *mask, double cg_tol, int32_t cg
                  double *loss, double *dlogeps, double *dtheta, int32_t *cg_it) {
     clear_error();
     if (!p || !mask || !loss || !dlogeps || !dtheta) return fail(PASC_ERR_ARG, "adjoint: NULL argument");
-    if (p->xdim != 1 || p->graph) return fail(PASC_ERR_ARG, "adjoint: 1-D signals (xdim == 1) only");
     if (!p->solved) return fail(PASC_ERR_STATE, "adjoint: solve the gamma problem first");
     if (p->b_set || !p->has_data) return fail(PASC_ERR_STATE, "adjoint: needs the data term (no explicit b)");
     if (p->fem_type != PASC_FEM_NONE) return fail(PASC_ERR_ARG, "adjoint: not for FEM-reduced problems");

@@ -44,48 +44,59 @@ class Context:
         self.close()
 
 
+def _problem_desc(x, K, T, R, xdim, scalef, precision, heldout, grid_width, fem_reduce,
+                  fem_type, graph):
+    """A pasc_problem_desc and the arrays it points into (kept alive by the caller).
+    x: data [T][R][xdim] (None: no data term, T and R given).  The graph of the R nodes is the
+    4-neighbour grid of width grid_width (T defaults to 1, an image), a CSR adjacency
+    graph = (nbr_ptr[R+1], nbr_idx), or none (R unconnected time chains)."""
+    keep = {}
+    if graph is not None:
+        keep["graph"] = (np.ascontiguousarray(graph[0], dtype=np.int64),
+                         np.ascontiguousarray(graph[1], dtype=np.int32))
+        R = keep["graph"][0].size - 1
+    if x is None:
+        if T is None:
+            raise ValueError("x=None needs T (and R / grid_width)")
+        keep["x"] = None
+        n = T * R
+    else:
+        keep["x"] = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        n = keep["x"].size // xdim
+        if grid_width > 0 and graph is None:
+            T = 1 if T is None else T
+            R = n // T
+        elif T is None:
+            T = n // R
+    assert T * R == n, "data size must be T*R*xdim"
+    keep["heldout"] = None if heldout is None else np.ascontiguousarray(heldout, dtype=np.uint8)
+    d = capi.ProblemDesc()
+    d.T, d.R, d.K, d.xdim = T, R, K, xdim
+    d.scalef, d.precision = int(scalef), precision
+    d.x = dptr(keep["x"]) if keep["x"] is not None else None
+    d.heldout = u8ptr(keep["heldout"]) if keep["heldout"] is not None else None
+    d.grid_width = grid_width
+    d.fem_reduce, d.fem_type = float(fem_reduce), int(fem_type)
+    if graph is not None:
+        d.nbr_ptr = keep["graph"][0].ctypes.data_as(C.POINTER(C.c_int64))
+        d.nbr_idx = keep["graph"][1].ctypes.data_as(C.POINTER(C.c_int32))
+    return d, keep, T, R, n
+
+
 class Problem:
     def __init__(self, ctx: Context, x, K: int, T: int | None = None, R: int = 1, xdim: int = 1,
                  scalef: bool = True, precision: int = capi.FP64, heldout=None,
                  grid_width: int = 0, epssqr: float = 1.0, fem_reduce: float = 1.0,
                  fem_type: int = capi.FEM_NONE, graph=None):
-        """x: data [T][R][xdim] (None: no data term, T and R given).  The graph of the R nodes
-        is the 4-neighbour grid of width grid_width (T defaults to 1, an image), a CSR adjacency
-        graph = (nbr_ptr[R+1], nbr_idx), or none (R unconnected time chains)."""
+        """One gamma QP (GraphH1FEMModel + TSData); arguments as in _problem_desc."""
         self.ctx = ctx
-        self._graph = None
-        if graph is not None:
-            self._graph = (np.ascontiguousarray(graph[0], dtype=np.int64),
-                           np.ascontiguousarray(graph[1], dtype=np.int32))
-            R = self._graph[0].size - 1
-        if x is None:  # no data term: the QP is defined by an explicit b
-            if T is None:
-                raise ValueError("x=None needs T (and R / grid_width)")
-            self.x = None
-            n = T * R
-        else:
-            self.x = np.ascontiguousarray(x, dtype=np.float64).ravel()
-            n = self.x.size // xdim
-            if grid_width > 0 and graph is None:
-                T = 1 if T is None else T
-                R = n // T
-            elif T is None:
-                T = n // R
-        assert T * R == n, "data size must be T*R*xdim"
+        d, self._keep, T, R, n = _problem_desc(x, K, T, R, xdim, scalef, precision, heldout,
+                                               grid_width, fem_reduce, fem_type, graph)
+        self.x = self._keep["x"]
+        self.heldout = self._keep["heldout"]
         self.T, self.R, self.K, self.xdim = T, R, K, xdim
         self.nodes = n
         self.grid_width = grid_width
-        self.heldout = None if heldout is None else np.ascontiguousarray(heldout, dtype=np.uint8)
-        d = capi.ProblemDesc()
-        d.T, d.R, d.K, d.xdim = self.T, self.R, K, xdim
-        d.scalef, d.precision = int(scalef), precision
-        d.x = dptr(self.x) if self.x is not None else None
-        d.heldout = u8ptr(self.heldout) if self.heldout is not None else None
-        d.grid_width = grid_width
-        d.fem_reduce, d.fem_type = float(fem_reduce), int(fem_type)
-        if self._graph is not None:
-            d.nbr_ptr = self._graph[0].ctypes.data_as(C.POINTER(C.c_int64))
-            d.nbr_idx = self._graph[1].ctypes.data_as(C.POINTER(C.c_int32))
         h = C.c_void_p()
         check(lib().pasc_problem_create(ctx.h, C.byref(d), C.byref(h)))
         self.h = h
@@ -176,7 +187,7 @@ class Problem:
     def adjoint(self, mask, cg_tol: float = 1e-10, cg_maxit: int = 10000):
         m = np.ascontiguousarray(mask, dtype=np.uint8)
         loss, dle = C.c_double(), C.c_double()
-        dth = np.empty(self.K)
+        dth = np.empty(self.K * self.xdim)
         it = C.c_int32()
         check(lib().pasc_adjoint(self.h, u8ptr(m), cg_tol, cg_maxit, C.byref(loss), C.byref(dle),
                                  dptr(dth), C.byref(it)))
@@ -240,6 +251,52 @@ class Batch:
     def get_gamma(self, s: int, e: int, layout: int = capi.LAYOUT_TMAJOR):
         g = np.empty(self.xs[s].size * self.Ks[s])
         check(lib().pasc_batch_get_gamma(self.h, s, e, dptr(g), layout))
+        return g
+
+
+class ProblemBatch:
+    """One problem (any graph, xdim, held-out mask) solved for several epsilon values at once."""
+
+    def __init__(self, ctx: Context, x, K: int, theta, eps, precision: int = capi.FP32_BULK,
+                 **problem_args):
+        self.ctx = ctx
+        d, self._keep, T, R, n = _problem_desc(x, K, problem_args.pop("T", None),
+                                               problem_args.pop("R", 1),
+                                               problem_args.pop("xdim", 1),
+                                               problem_args.pop("scalef", True), precision,
+                                               problem_args.pop("heldout", None),
+                                               problem_args.pop("grid_width", 0), 1.0,
+                                               capi.FEM_NONE, problem_args.pop("graph", None))
+        assert not problem_args, f"unknown arguments {list(problem_args)}"
+        self.theta = np.ascontiguousarray(theta, dtype=np.float64).ravel()
+        self.eps = np.ascontiguousarray(eps, dtype=np.float64).ravel()
+        self.K, self.nodes = K, n
+        h = C.c_void_p()
+        check(lib().pasc_batch_create_problem(ctx.h, C.byref(d), dptr(self.theta), self.eps.size,
+                                              dptr(self.eps), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().pasc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, **params):
+        p = capi.default_params(**params)
+        res = (capi.InstanceResult * self.eps.size)()
+        t = C.c_double()
+        check(lib().pasc_batch_solve(self.h, C.byref(p), res, C.byref(t)))
+        return list(res), t.value
+
+    def get_gamma(self, e: int, layout: int = capi.LAYOUT_TMAJOR):
+        g = np.empty(self.nodes * self.K)
+        check(lib().pasc_batch_get_gamma(self.h, 0, e, dptr(g), layout))
         return g
 
 

@@ -28,10 +28,12 @@ def _loss(P, x, mask, theta, log_eps, params):
     return float(np.sum(res ** 2)), r
 
 
-def test_adjoint_matches_central_differences(ctx):
+@pytest.mark.parametrize("T", [10000, 1000000])
+def test_adjoint_matches_central_differences(ctx, T):
+    """The differentiation workload (T = 1e6 at full size; levels {0, 1.5, 3}, switch 1e-3,
+    noise 0.8, seed 3, 10% held out, epsilon 30) and a T = 1e4 slice of it."""
     from pasc_inference_amd import capi
     from pasc_inference_amd.solver import Problem
-    T = 10000
     x, mask, theta, eps = _case(T)
     K = theta.size
     P = Problem(ctx, x, K, precision=capi.FP64, heldout=mask, epssqr=eps)
@@ -45,7 +47,10 @@ def test_adjoint_matches_central_differences(ctx):
     Jm, _ = _loss(P, x, mask, theta, np.log(eps) - h, params)
     fd = (Jp - Jm) / (2 * h)
     print(f"FD dlogeps={fd!r}")
-    assert abs(fd - dle) <= 1e-4 * max(abs(fd), 1e-3 * J)
+    # J is piecewise smooth (the active face changes where a gamma_kt reaches or leaves 0); at
+    # T = 1e6 a step of 1e-4 crosses a few such changes, so the bound there is 5e-4
+    tol = 1e-4 if T <= 10000 else 5e-4
+    assert abs(fd - dle) <= tol * max(abs(fd), 1e-3 * J)
     for k in range(K):
         tp, tm = theta.copy(), theta.copy()
         tp[k] += h
@@ -54,7 +59,7 @@ def test_adjoint_matches_central_differences(ctx):
         Jm, _ = _loss(P, x, mask, tm, np.log(eps), params)
         fdk = (Jp - Jm) / (2 * h)
         print(f"FD dtheta[{k}]={fdk!r} adjoint={dth[k]!r}")
-        assert abs(fdk - dth[k]) <= 1e-4 * max(abs(fdk), 1e-3 * J)
+        assert abs(fdk - dth[k]) <= tol * max(abs(fdk), 1e-3 * J)
 
 
 def _torch_spg_loss(x, mask, theta, log_eps, T, K, iters):
@@ -118,7 +123,7 @@ def test_adjoint_matches_unrolled_spg(ctx):
     from pasc_inference_amd import capi
     from pasc_inference_amd.solver import Problem
     import torch
-    T = 2000
+    T = 10000
     x, mask, theta, eps = _case(T, seed=5, frac=0.1)
     K = theta.size
     P = Problem(ctx, x, K, precision=capi.FP64, heldout=mask, epssqr=eps)

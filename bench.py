@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--profile-run", action="store_true",
                     help="only a fixed number of fused passes of the T=1e7 solve in each "
                          "precision plus a copy kernel (for rocprofv3 traces and counters)")
+    ap.add_argument("--scan-profile", action="store_true",
+                    help="run only the batched scan (for rocprofv3 traces of the batched engine)")
     ap.add_argument("--tune-cache", default=None,
                     help="autotuner cache file: launch shapes recorded there are reused, new ones appended")
     ap.add_argument("--verbose", action="store_true")
@@ -115,9 +117,8 @@ def run_c2(args, ctx):
     P32.set_theta(theta)
     if not args.no_autotune:
         P32.set_gamma_uniform()
-        tune["fp32"] = P32.autotune(args.tune_cache)
-        log(args, f"autotune fp32: block={tune['fp32'][0]} chunk={tune['fp32'][1]} "
-                  f"prefetch={tune['fp32'][2]} slots={tune['fp32'][4]} {tune['fp32'][3]:.1f} us/pass")
+        tune["fp32_bulk"] = P32.autotune(args.tune_cache)
+        log(args, f"autotune fp32-bulk: {tune['fp32_bulk']}")
     t32, r32 = _timed_solves(args, P32, fast, "fp32-bulk solve")
     g32 = P32.get_gamma()
     f32, kkt32 = P32.objective()
@@ -134,8 +135,7 @@ def run_c2(args, ctx):
     if not args.no_autotune:
         P64.set_gamma_uniform()
         tune["fp64"] = P64.autotune(args.tune_cache)
-        log(args, f"autotune fp64: block={tune['fp64'][0]} chunk={tune['fp64'][1]} "
-                  f"prefetch={tune['fp64'][2]} slots={tune['fp64'][4]} {tune['fp64'][3]:.1f} us/pass")
+        log(args, f"autotune fp64: {tune['fp64']}")
     t64g, r64g = _timed_solves(args, P64, fast, "fp64 graph solve")
     g64 = P64.get_gamma()
     f64, kkt64 = P64.objective()
@@ -196,6 +196,8 @@ def run_scan(args, ctx):
     params = dict(maxit=args.scan_maxit, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt,
                   check_every=64, use_graph=0 if args.no_graph else 1, kkt_every=4)
     B = Batch(ctx, xs, K, ths, eps, precision=capi.FP32_BULK)
+    if not args.no_autotune:
+        log(args, f"autotune batch: {B.autotune(args.tune_cache)}")
     ctx.synchronize()
     t0 = time.perf_counter()
     res, tdev = B.solve(**params)
@@ -205,6 +207,9 @@ def run_scan(args, ctx):
     its = np.array([r.it for r in res])
     B.close()
     log(args, f"batched scan: {tb:.3f}s for {n_inst} solves, {ok} certified, it max {its.max()}")
+    if args.scan_profile:
+        return dict(batched_wall_s=tb, n_inst=n_inst, certified=ok, it_max=int(its.max()),
+                    it_mean=float(its.mean()))
     # plain FP64 per-solve path on a deterministic sample of the same instances
     rng = np.random.default_rng(12345)
     pick = rng.choice(n_inst, size=min(args.plain_sample, n_inst), replace=False)
@@ -295,6 +300,11 @@ def main():
     if args.profile_run:
         profile_run(args)
         return
+    if args.scan_profile:
+        from pasc_inference_amd.solver import Context
+        with Context(0) as ctx:
+            print(json.dumps(run_scan(args, ctx)))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path "
               f"(requested --gpus {args.gpus})", file=sys.stderr)
@@ -346,9 +356,7 @@ def main():
         "iterations": {"fp64_graph": r64g.it, "fp32_bulk": [r32.it_fp32, r32.it_fp64],
                        "plain_fp64": c2["r64"].it},
         "speedup_vs_plain_fp64": float(np.mean(c2["tp64"])) / best,
-        "autotune": {k: {"block": v[0], "chunk": v[1], "prefetch": v[2], "us_per_pass": v[3],
-                         "max_slots": v[4]}
-                     for k, v in c2["tune"].items()},
+        "autotune": c2["tune"],
         # whole solve against the roofline, 6*n*s algorithmic bytes per SPG iteration
         "solve_roofline_frac": solve_bytes / best / (HBM_PEAK_GBS * 1e9),
         "errors_vs_fp64_reference_solve": {"fp64_graph": c2["err64"], "fp32_bulk": c2["err32"],

@@ -297,22 +297,34 @@ int pasc_device_copy_bandwidth(pasc_ctx *ctx, int64_t bytes, int32_t reps, doubl
 /* ---- launch shape of the fused 1-D pass ---- */
 typedef struct {
     int32_t block;       /* threads per workgroup: 128, 256 or 512 */
-    int32_t chunk;       /* time steps per thread: FP32-bulk 4 or 8, FP64 2 or 4 */
-    int32_t prefetch;    /* 1: a workgroup loads its next tile while it computes the current one */
-    int32_t max_slots;   /* partial-sum slots per instance, 0 = 2048; each slot is one
+    int32_t chunk;       /* time steps per thread: FP32 passes 4 or 8, FP64 passes 2 or 4 */
+    int32_t prefetch;    /* 1: asynchronous LDS staging of a workgroup's next tile */
+    int32_t max_slots;   /* partial-sum slots per instance at most (0 = 2048); each slot is one
                             workgroup's run of consecutive tiles */
-    double us_per_pass;  /* pasc_autotune: measured average pass time (0 when taken from a cache) */
+    int32_t min_tiles;   /* tiles per slot at least (0 = 1); staging overlaps from 2 on */
+    double us_per_pass;  /* pasc_autotune: measured average pass time (0 when from a cache) */
+} pasc_pass_shape;
+
+typedef struct {
+    pasc_pass_shape main;    /* FP64 problems: every pass; FP32-bulk: the FP32 phase */
+    pasc_pass_shape polish;  /* FP32-bulk: the FP64 polish passes (ignored for FP64 problems);
+                                polish.block == 0 derives it from main (chunk / 2) */
 } pasc_launch_shape;
-/* Pin the launch shape of later solves (the iterate is kept). */
+
+/* Pin the launch shapes of later solves (the iterate is kept). */
 int pasc_set_launch_shape(pasc_problem *p, const pasc_launch_shape *shape);
 
 /* ---- autotuner ----
  * Times the fused SPG pass for the problem's (K, precision, T) over the candidate launch
- * shapes (block x chunk x prefetch) and keeps the fastest for later solves.
+ * shapes (block x chunk x prefetch x slots) and keeps the fastest for later solves; for an
+ * FP32-bulk problem the FP32 phase and the FP64 polish are tuned separately.
  * The choice is cached per context.  With a non-NULL cache_path, a choice recorded there for
- * the same key (device, K, precision, log2 T) by an earlier process is reused without timing,
- * and every call appends its choice to the file (one JSON object per line). */
+ * the same key by an earlier process is reused without timing, and every call appends its
+ * choice to the file (one JSON object per line). */
 int pasc_autotune(pasc_problem *p, const char *cache_path, pasc_launch_shape *shape);
+/* the same for a batch (all instances share one shape per phase) */
+int pasc_batch_autotune(pasc_batch *b, const char *cache_path, pasc_launch_shape *shape);
+int pasc_batch_set_launch_shape(pasc_batch *b, const pasc_launch_shape *shape);
 
 /* ---- synthetic data (deterministic SplitMix64 + Box-Muller generator) ---- */
 int pasc_generate_markov(uint64_t seed, int64_t T, int32_t K, const double *levels,

@@ -78,9 +78,16 @@ class SeriesDesc(C.Structure):
                 ("theta", C.POINTER(C.c_double))]
 
 
-class LaunchShape(C.Structure):
+class PassShape(C.Structure):
     _fields_ = [("block", C.c_int32), ("chunk", C.c_int32), ("prefetch", C.c_int32),
-                ("max_slots", C.c_int32), ("us_per_pass", C.c_double)]
+                ("max_slots", C.c_int32), ("min_tiles", C.c_int32), ("us_per_pass", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class LaunchShape(C.Structure):
+    _fields_ = [("main", PassShape), ("polish", PassShape)]
 
 
 class InstanceResult(C.Structure):
@@ -142,6 +149,8 @@ _SIGS = {
     "pasc_device_copy_bandwidth": ([vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)], C.c_int),
     "pasc_autotune": ([vp, C.c_char_p, C.POINTER(LaunchShape)], C.c_int),
     "pasc_set_launch_shape": ([vp, C.POINTER(LaunchShape)], C.c_int),
+    "pasc_batch_autotune": ([vp, C.c_char_p, C.POINTER(LaunchShape)], C.c_int),
+    "pasc_batch_set_launch_shape": ([vp, C.POINTER(LaunchShape)], C.c_int),
     "pasc_generate_markov": ([C.c_uint64, C.c_int64, C.c_int32, C.POINTER(C.c_double),
                               C.c_double, C.c_double, C.POINTER(C.c_double),
                               C.POINTER(C.c_int32)], C.c_int),

@@ -915,13 +915,114 @@ int pasc_set_profiling(pasc_problem *p, int enable) {
     return PASC_OK;
 }
 
+static PassShape to_pass(const pasc_pass_shape &s) {
+    return PassShape{s.block, s.chunk, s.prefetch, s.max_slots > 0 ? s.max_slots : 2048,
+                     s.min_tiles > 0 ? s.min_tiles : 1};
+}
+
+static pasc_pass_shape from_pass(const PassShape &s, double us) {
+    return pasc_pass_shape{s.block, s.chunk, s.prefetch, s.max_slots, s.min_tiles, us};
+}
+
+// FP32 / FP64 phase shapes of an engine from the ABI struct
+static void shapes_of(const Engine &e, const pasc_launch_shape &sh, PassShape &s32, PassShape &s64) {
+    if (e.precision == PASC_FP32_BULK) {
+        s32 = to_pass(sh.main);
+        if (sh.polish.block > 0) {
+            s64 = to_pass(sh.polish);
+        } else {
+            s64 = s32;
+            s64.chunk = s32.chunk / 2;
+        }
+    } else {
+        s32 = e.sh[0];
+        s64 = to_pass(sh.main);
+    }
+}
+
+static int set_shape(Engine &e, const pasc_launch_shape *shape) {
+    if (e.stencil != STENCIL_TIME) return fail(PASC_ERR_ARG, "set_launch_shape: the graph path has a fixed shape");
+    PassShape s32, s64;
+    shapes_of(e, *shape, s32, s64);
+    return engine_set_shape(e, s32, s64);
+}
+
+// Autotune with the per-context and on-file caches (one JSON object per line, last match wins).
+static int autotune_cached(Engine &e, Ctx &c, const std::string &key, int64_t size,
+                           const char *cache_path, pasc_launch_shape *shape) {
+    pasc_launch_shape sh;
+    std::memset(&sh, 0, sizeof(sh));
+    bool hit = false;
+    for (auto &kv : c.tune)
+        if (kv.first == key) {
+            sh = kv.second;
+            sh.main.us_per_pass = sh.polish.us_per_pass = 0.0;
+            hit = true;
+        }
+    if (!hit && cache_path) {
+        FILE *f = fopen(cache_path, "r");
+        if (f) {
+            char line[1024], want[200];
+            snprintf(want, sizeof(want), "\"key\": \"%s\"", key.c_str());
+            while (fgets(line, sizeof(line), f)) {
+                if (!strstr(line, want)) continue;
+                const char *ps = strstr(line, "\"shape\": [");
+                int v[10];
+                if (ps && sscanf(ps + 10, "[%d, %d, %d, %d, %d, %d, %d, %d, %d, %d]", &v[0], &v[1],
+                                 &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9]) == 10) {
+                    sh.main = pasc_pass_shape{v[0], v[1], v[2], v[3], v[4], 0.0};
+                    sh.polish = pasc_pass_shape{v[5], v[6], v[7], v[8], v[9], 0.0};
+                    hit = true;
+                }
+            }
+            fclose(f);
+            if (hit) c.tune.push_back({key, sh});
+        }
+    }
+    int rc;
+    if (hit) {
+        PassShape s32, s64;
+        shapes_of(e, sh, s32, s64);
+        rc = engine_set_shape(e, s32, s64);
+    } else {
+        PassShape s32, s64;
+        double us[2];
+        rc = engine_autotune(e, &s32, &s64, us);
+        if (rc == PASC_OK) {
+            if (e.precision == PASC_FP32_BULK) {
+                sh.main = from_pass(s32, us[0]);
+                sh.polish = from_pass(s64, us[1]);
+            } else {
+                sh.main = from_pass(s64, us[1]);
+                sh.polish = from_pass(s64, us[1]);
+            }
+            c.tune.push_back({key, sh});
+        }
+    }
+    if (rc) return rc;
+    if (shape) *shape = sh;
+    if (cache_path) {
+        FILE *f = fopen(cache_path, "a");
+        if (f) {
+            fprintf(f, "{\"key\": \"%s\", \"size\": %lld, \"shape\": [%d, %d, %d, %d, %d, %d, %d, %d, %d, %d], "
+                       "\"us_per_pass\": [%.3f, %.3f], \"cached\": %d}\n",
+                    key.c_str(), (long long)size, sh.main.block, sh.main.chunk, sh.main.prefetch,
+                    sh.main.max_slots, sh.main.min_tiles, sh.polish.block, sh.polish.chunk,
+                    sh.polish.prefetch, sh.polish.max_slots, sh.polish.min_tiles,
+                    sh.main.us_per_pass, sh.polish.us_per_pass, hit ? 1 : 0);
+            fclose(f);
+        }
+    }
+    return PASC_OK;
+}
+
 int pasc_set_launch_shape(pasc_problem *p, const pasc_launch_shape *shape) {
     clear_error();
     if (!p || !shape) return fail(PASC_ERR_ARG, "set_launch_shape: NULL argument");
     if (p->graph) return fail(PASC_ERR_ARG, "set_launch_shape: the graph path has a fixed shape");
     int rc = use_device(p->ctx);
     if (rc) return rc;
-    return engine_set_shape(p->e, shape->block, shape->chunk, shape->prefetch, shape->max_slots);
+    return set_shape(p->e, shape);
 }
 
 int pasc_autotune(pasc_problem *p, const char *cache_path, pasc_launch_shape *shape) {
@@ -939,64 +1040,37 @@ int pasc_autotune(pasc_problem *p, const char *cache_path, pasc_launch_shape *sh
         rc = fem_before_solve(p);  // the coarse QP the autotuner times
         if (rc) return rc;
     }
-    pasc_launch_shape sh{0, 0, 0, 0, 0.0};
-    // the choice is cached per context, keyed by device, K, precision and log2(T)
-    char key[128];
+    const int64_t n = p->e.h_desc[0].T;
+    // keyed by device, K, precision and log2(T)
+    char key[160];
     snprintf(key, sizeof(key), "%s/K%d/p%d/T2^%d", p->ctx->c.arch.c_str(), p->K, p->precision,
-             (int)std::floor(std::log2((double)p->T)));
-    bool hit = false;
-    for (auto &kv : p->ctx->c.tune)
-        if (kv.first == key) {
-            sh = kv.second;
-            sh.us_per_pass = 0.0;
-            hit = true;
-        }
-    if (!hit && cache_path) {
-        // a cache file written by an earlier process: the last line with this key wins
-        FILE *f = fopen(cache_path, "r");
-        if (f) {
-            char line[512], want[160];
-            snprintf(want, sizeof(want), "\"key\": \"%s\"", key);
-            while (fgets(line, sizeof(line), f)) {
-                if (!strstr(line, want)) continue;
-                const char *pb = strstr(line, "\"block\": ");
-                const char *pc = strstr(line, "\"chunk\": ");
-                const char *pp = strstr(line, "\"prefetch\": ");
-                const char *ps = strstr(line, "\"max_slots\": ");
-                int b2 = 0, c2 = 0, p2 = 0, s2 = 0;
-                if (pb && pc && sscanf(pb + 9, "%d", &b2) == 1 && sscanf(pc + 9, "%d", &c2) == 1) {
-                    if (!(pp && sscanf(pp + 12, "%d", &p2) == 1)) p2 = 0;
-                    if (!(ps && sscanf(ps + 13, "%d", &s2) == 1)) s2 = 0;
-                    sh = pasc_launch_shape{b2, c2, p2, s2, 0.0};
-                    hit = true;
-                }
-            }
-            fclose(f);
-            if (hit) p->ctx->c.tune.push_back({key, sh});
-        }
-    }
-    if (hit) {
-        rc = engine_set_shape(p->e, sh.block, sh.chunk, sh.prefetch, sh.max_slots);
-    } else {
-        int bl = 0, ch = 0, pf = 0, ms = 0;
-        double us = 0.0;
-        rc = engine_autotune(p->e, &bl, &ch, &pf, &ms, &us);
-        sh = pasc_launch_shape{bl, ch, pf, ms, us};
-        if (rc == PASC_OK) p->ctx->c.tune.push_back({key, sh});
-    }
+             (int)std::floor(std::log2((double)n)));
+    return autotune_cached(p->e, p->ctx->c, key, n, cache_path, shape);
+}
+
+int pasc_batch_set_launch_shape(pasc_batch *b, const pasc_launch_shape *shape) {
+    clear_error();
+    if (!b || !shape) return fail(PASC_ERR_ARG, "batch_set_launch_shape: NULL argument");
+    int rc = use_device(b->ctx);
     if (rc) return rc;
-    if (shape) *shape = sh;
-    if (cache_path) {
-        FILE *f = fopen(cache_path, "a");
-        if (f) {
-            fprintf(f, "{\"key\": \"%s\", \"T\": %lld, \"block\": %d, \"chunk\": %d, \"prefetch\": %d, "
-                       "\"max_slots\": %d, \"us_per_pass\": %.3f, \"cached\": %d}\n",
-                    key, (long long)p->T, sh.block, sh.chunk, sh.prefetch, sh.max_slots,
-                    sh.us_per_pass, hit ? 1 : 0);
-            fclose(f);
-        }
-    }
-    return PASC_OK;
+    return set_shape(b->e, shape);
+}
+
+int pasc_batch_autotune(pasc_batch *b, const char *cache_path, pasc_launch_shape *shape) {
+    clear_error();
+    if (!b) return fail(PASC_ERR_ARG, "batch_autotune: NULL batch");
+    if (b->e.stencil != STENCIL_TIME) return fail(PASC_ERR_ARG, "batch_autotune: the graph path has a fixed shape");
+    int rc = use_device(b->ctx);
+    if (rc) return rc;
+    int64_t elems = 0;
+    for (const auto &d : b->e.h_desc) elems += d.T * d.K;
+    // keyed by device, precision, instance count and log2 of the total elements
+    char key[160];
+    snprintf(key, sizeof(key), "%s/batch%d/p%d/E2^%d", b->ctx->c.arch.c_str(), b->e.n_inst,
+             b->e.precision, (int)std::floor(std::log2((double)std::max<int64_t>(elems, 1))));
+    rc = engine_fill_uniform(b->e);  // the scan starts from gamma0 = 1/K
+    if (rc) return rc;
+    return autotune_cached(b->e, b->ctx->c, key, elems, cache_path, shape);
 }
 
 }  // extern "C"

@@ -59,24 +59,28 @@ static int launch_pass_1d(bool f64, int K, const PassArgs &a, int grid, int bloc
 // ------------------------------------------------------------------------------------------
 // allocation
 // ------------------------------------------------------------------------------------------
-// time steps per tile of the pass kernels: the FP64 phase of an FP32-bulk problem runs the
-// double instantiation with half the time steps per thread, so the tiling depends on the phase
+// time steps per tile of the pass kernels in a phase (shape index 1: FP64 passes)
 static int tile_len(const Engine &e, bool f64_phase) {
     if (e.stencil != STENCIL_TIME) return 256;
-    const int ch = f64_phase ? e.chunk64() : e.chunk;
-    const int ht = (3 + ch - 1) / ch;  // halo threads per side (see tile_steps)
-    return (e.block - 2 * ht) * ch;
+    const PassShape &sh = e.sh[f64_phase ? 1 : 0];
+    const int ht = (3 + sh.chunk - 1) / sh.chunk;  // halo threads per side (see tile_steps)
+    return (sh.block - 2 * ht) * sh.chunk;
 }
 
-static int engine_retile(Engine &e, int tl) {
+static int engine_retile(Engine &e, bool f64_phase) {
+    const PassShape &sh = e.sh[f64_phase ? 1 : 0];
+    const int tl = tile_len(e, f64_phase);
     e.cur_tile = tl;
+    e.cur_phase = f64_phase ? 1 : 0;
     int slots = 0;
     for (auto &d : e.h_desc) {
         // at most max_slots (default 2048) partial slots per instance: enough workgroups to fill
-        // the device, few enough that the step kernel reduces them quickly
+        // the device, few enough that the step kernel reduces them quickly; at least min_tiles
+        // tiles per slot
         const int64_t raw = (d.T + tl - 1) / tl;
-        const int64_t ms = std::max(e.max_slots, 1);
-        d.msub = (int32_t)std::max<int64_t>(1, (raw + ms - 1) / ms);
+        const int64_t ms = std::max(sh.max_slots, 1);
+        d.msub = (int32_t)std::max<int64_t>(std::max(sh.min_tiles, 1), (raw + ms - 1) / ms);
+        d.msub = (int32_t)std::min<int64_t>(d.msub, std::max<int64_t>(raw, 1));
         d.tile0 = slots;
         d.ntiles = (int32_t)((raw + d.msub - 1) / d.msub);
         slots += d.ntiles;
@@ -125,8 +129,8 @@ int engine_alloc(Engine &e, Ctx *ctx, int precision, int stencil, int32_t grid_w
     e.h_desc = descs;
     e.n_inst = (int)descs.size();
     e.data_elems = data_elems;
-    e.chunk = precision == PASC_FP32_BULK ? 4 : 2;
-    e.block = 256;
+    e.sh[0] = PassShape{256, 4, 0, 2048, 1};
+    e.sh[1] = PassShape{256, 2, 0, 2048, 1};
     int64_t se = 0;
     for (auto &d : descs) se = std::max(se, d.goff + (int64_t)d.K * d.ld);
     e.state_elems = round_up(std::max<int64_t>(se, 64), 64);
@@ -169,7 +173,7 @@ int engine_alloc(Engine &e, Ctx *ctx, int precision, int stencil, int32_t grid_w
         e.groups.push_back(g);
     }
     if (e.groups.size() > (size_t)MAX_GROUPS) return fail(PASC_ERR_ARG, "too many K groups");
-    return engine_retile(e, tile_len(e, precision != PASC_FP32_BULK));
+    return engine_retile(e, precision != PASC_FP32_BULK);
 }
 
 void engine_free(Engine &e) {
@@ -386,8 +390,8 @@ static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<in
             const void *bb = ph.f64 ? e.d_b64 : e.d_b32;
             PassArgs a{g0, g1, x, e.d_desc, e.d_state, e.d_part, gv, ph.kkt_every,
                        ph.sp.fx_identity ? 0 : 1, bb};
-            rc = launch_pass_1d(ph.f64, g.K, a, grids[gi], e.block, ph.f64 ? e.chunk64() : e.chunk,
-                                e.prefetch, s);
+            const PassShape &sh = e.sh[ph.f64 ? 1 : 0];
+            rc = launch_pass_1d(ph.f64, g.K, a, grids[gi], sh.block, sh.chunk, sh.prefetch, s);
         } else {
             void *d0 = ph.f64 && e.precision == PASC_FP32_BULK ? e.d_dir64[0] : e.d_dir[0];
             void *d1 = ph.f64 && e.precision == PASC_FP32_BULK ? e.d_dir64[1] : e.d_dir[1];
@@ -420,9 +424,8 @@ static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<in
 
 static int run_phase(Engine &e, const PhaseCfg &ph, const pasc_spg_params &p, int max_passes) {
     hipStream_t s = S_(e);
-    const int tl = tile_len(e, ph.f64);
-    if (tl != e.cur_tile) {
-        int rc0 = engine_retile(e, tl);
+    if (e.cur_phase != (ph.f64 ? 1 : 0) || tile_len(e, ph.f64) != e.cur_tile) {
+        int rc0 = engine_retile(e, ph.f64);
         if (rc0) return rc0;
     }
     // initial scan (active tiles of every group)
@@ -949,33 +952,41 @@ int engine_fill_uniform(Engine &e) {
     return PASC_OK;
 }
 
-int engine_set_shape(Engine &e, int block, int chunk, int prefetch, int max_slots) {
+static int check_shape(const PassShape &s, bool f32) {
+    const bool ok_chunk = f32 ? (s.chunk == 4 || s.chunk == 8) : (s.chunk == 2 || s.chunk == 4);
+    if (!(s.block == 128 || s.block == 256 || s.block == 512) || !ok_chunk ||
+        (s.prefetch != 0 && s.prefetch != 1))
+        return fail(PASC_ERR_ARG, "no fused %s pass for block %d chunk %d prefetch %d",
+                    f32 ? "FP32" : "FP64", s.block, s.chunk, s.prefetch);
+    if (s.max_slots < 1 || s.min_tiles < 1) return fail(PASC_ERR_ARG, "max_slots and min_tiles must be >= 1");
+    return PASC_OK;
+}
+
+int engine_set_shape(Engine &e, const PassShape &s32, const PassShape &s64) {
     const bool f32 = e.precision == PASC_FP32_BULK;
-    const bool ok_chunk = f32 ? (chunk == 4 || chunk == 8) : (chunk == 2 || chunk == 4);
-    if (!(block == 128 || block == 256 || block == 512) || !ok_chunk || (prefetch != 0 && prefetch != 1))
-        return fail(PASC_ERR_ARG, "no fused pass for block %d chunk %d prefetch %d", block, chunk,
-                    prefetch);
-    if (max_slots < 0) return fail(PASC_ERR_ARG, "max_slots < 0");
-    e.prefetch = prefetch;
-    e.max_slots = max_slots > 0 ? max_slots : 2048;
+    int rc = f32 ? check_shape(s32, true) : PASC_OK;
+    if (rc == PASC_OK) rc = check_shape(s64, false);
+    if (rc) return rc;
     std::vector<std::vector<double>> saved(e.n_inst);
     for (int i = 0; i < e.n_inst; i++) {
-        int rc = download_rows(e, i, saved[i]);
+        rc = download_rows(e, i, saved[i]);
         if (rc) return rc;
     }
-    e.block = block;
-    e.chunk = chunk;
-    int rc = engine_retile(e, tile_len(e, !f32));
+    if (f32) e.sh[0] = s32;
+    e.sh[1] = s64;
+    rc = engine_retile(e, !f32);
     for (int i = 0; i < e.n_inst && rc == PASC_OK; i++) rc = upload_rows(e, i, saved[i]);
     return rc;
 }
 
-// Self-written autotuner: times the fused pass of this problem for every compiled
+// Self-written autotuner: times the fused pass of this problem (or batch) for every compiled
 // (threads per block, time steps per thread, prefetch) triple, and with prefetch also for
-// 2048 / 1024 / 512 partial slots per instance (more tiles per workgroup lengthen its
-// pipeline) -- 20 passes each from the current iterate with the KKT residual on every 4th
-// pass, hipEvents around every launch -- and keeps the fastest.  The iterate is restored.
-int engine_autotune(Engine &e, int *block, int *chunk, int *prefetch, int *max_slots, double *us) {
+// 2048 / 1024 / 512 partial slots per instance and for at least 4 tiles per slot (more tiles
+// per workgroup lengthen its pipeline) -- 20 passes each from the current iterate with the KKT
+// residual on every 4th pass, hipEvents around every launch.  An FP32-bulk problem runs 20 FP32
+// and 20 FP64 polish passes per candidate, and the fastest shape of each phase is kept.  The
+// iterate is restored afterwards.
+int engine_autotune(Engine &e, PassShape *s32, PassShape *s64, double us[2]) {
     if (e.stencil != STENCIL_TIME) return fail(PASC_ERR_ARG, "autotune: 1-D time graph only");
     std::vector<std::vector<double>> saved(e.n_inst);
     for (int i = 0; i < e.n_inst; i++) {
@@ -983,59 +994,50 @@ int engine_autotune(Engine &e, int *block, int *chunk, int *prefetch, int *max_s
         if (rc) return rc;
     }
     const bool f32 = e.precision == PASC_FP32_BULK;
-    const int chunks[2] = {f32 ? 4 : 2, f32 ? 8 : 4};
+    const int chunks32[2] = {4, 8}, chunks64[2] = {2, 4};
     const int blocks[3] = {128, 256, 512};
+    const int slot_opts[4][2] = {{2048, 1}, {1024, 1}, {512, 1}, {2048, 4}};
     pasc_spg_params p;
     pasc_spg_default_params(&p);
     p.maxit = 20;
     p.stop_mask = 0;
-    p.polish_maxit = 1;
+    p.polish_maxit = f32 ? 20 : 1;
     p.kkt_every = 4;
     const bool prof = e.profile;
     e.profile = true;
-    double best = 1e300;
-    int bb = e.block, bc = e.chunk, bp = e.prefetch, bs = e.max_slots;
-    const int slot_opts[3] = {2048, 1024, 512};
+    double best[2] = {1e300, 1e300};
+    PassShape bs[2] = {e.sh[0], e.sh[1]};
     for (int pi = 0; pi < 2; pi++)
-    for (int si = 0; si < (pi ? 3 : 1); si++)
-    for (int ci = 0; ci < 2; ci++) {
-        for (int bi = 0; bi < 3; bi++) {
-            e.block = blocks[bi];
-            e.chunk = chunks[ci];
-            e.prefetch = pi;
-            e.max_slots = slot_opts[si];
-            int rc = engine_retile(e, tile_len(e, !f32));
-            for (int i = 0; i < e.n_inst && rc == PASC_OK; i++) rc = upload_rows(e, i, saved[i]);
-            pasc_spg_result r;
-            if (rc == PASC_OK) rc = engine_solve(e, p, &r, nullptr, nullptr);
-            if (rc) {
-                e.profile = prof;
-                return rc;
-            }
-            const double t = f32 ? e.prof_ms[0] / std::max(e.prof_n[0], 1)
-                                 : e.prof_ms[1] / std::max(e.prof_n[1], 1);
-            if (t < best) {
-                best = t;
-                bb = e.block;
-                bc = e.chunk;
-                bp = e.prefetch;
-                bs = e.max_slots;
-            }
+    for (int si = 0; si < (pi ? 4 : 1); si++)
+    for (int ci = 0; ci < 2; ci++)
+    for (int bi = 0; bi < 3; bi++) {
+        const PassShape c32{blocks[bi], chunks32[ci], pi, slot_opts[si][0], slot_opts[si][1]};
+        const PassShape c64{blocks[bi], chunks64[ci], pi, slot_opts[si][0], slot_opts[si][1]};
+        e.sh[0] = c32;
+        e.sh[1] = c64;
+        int rc = engine_retile(e, !f32);
+        for (int i = 0; i < e.n_inst && rc == PASC_OK; i++) rc = upload_rows(e, i, saved[i]);
+        pasc_spg_result r;
+        if (rc == PASC_OK) rc = engine_solve(e, p, &r, nullptr, nullptr);
+        if (rc) {
+            e.profile = prof;
+            return rc;
         }
+        const double t32 = e.prof_n[0] ? e.prof_ms[0] / e.prof_n[0] : 1e300;
+        const double t64 = e.prof_n[1] ? e.prof_ms[1] / e.prof_n[1] : 1e300;
+        if (f32 && t32 < best[0]) { best[0] = t32; bs[0] = c32; }
+        if (t64 < best[1]) { best[1] = t64; bs[1] = c64; }
     }
     e.profile = prof;
-    e.block = bb;
-    e.chunk = bc;
-    e.prefetch = bp;
-    e.max_slots = bs;
-    int rc = engine_retile(e, tile_len(e, !f32));
+    e.sh[0] = bs[0];
+    e.sh[1] = bs[1];
+    int rc = engine_retile(e, !f32);
     for (int i = 0; i < e.n_inst && rc == PASC_OK; i++) rc = upload_rows(e, i, saved[i]);
     if (rc) return rc;
-    *block = bb;
-    *chunk = bc;
-    *prefetch = bp;
-    *max_slots = bs;
-    *us = best * 1e3;
+    *s32 = bs[0];
+    *s64 = bs[1];
+    us[0] = f32 ? best[0] * 1e3 : 0.0;
+    us[1] = best[1] * 1e3;
     return PASC_OK;
 }
 

@@ -86,6 +86,15 @@ struct StepParams {
 // blockgraphsparse.cpp (4-neighbour image grid or CSR adjacency, T >= 1), node-wise kernels
 enum StencilKind { STENCIL_TIME = 0, STENCIL_GRID = 1 };
 
+// Launch shape of the fused 1-D pass for one phase
+struct PassShape {
+    int block = 256;       // threads per block
+    int chunk = 2;         // time steps per thread (of the phase's type)
+    int prefetch = 0;      // 1: asynchronous LDS staging of the next tile
+    int max_slots = 2048;  // partial slots per instance at most
+    int min_tiles = 1;     // tiles per partial slot at least (lengthens the staging pipeline)
+};
+
 struct Ctx;
 
 // Engine: one or many instances sharing state buffers (k-major rows, padded).
@@ -131,16 +140,13 @@ struct Engine {
     std::vector<Group> groups;
     int32_t *d_nactive = nullptr;        // device counter of active instances
     int32_t *h_nactive = nullptr;        // pinned mirror
-    int block = 256;                     // threads per block of the fused 1-D pass
-    int chunk = 8;                       // time steps per thread (of the storage type)
-    int prefetch = 0;                    // 1: pass kernel loads the next tile ahead
-    int max_slots = 2048;                // partial slots per instance (tiles per slot follow)
+    PassShape sh[2];                     // [0] FP32 phase of FP32-bulk, [1] every FP64 pass
     int grid_cap = 0;                    // persistent grid cap (0: 32 blocks per CU)
     int cur_tile = 0;                    // time steps per tile the partial slots are sized for
+    int cur_phase = -1;                  // shape index the partial slots are sized for
     bool profile = false;                // hipEvents around every pass launch
     double prof_ms[2] = {0.0, 0.0};      // summed pass time per phase (FP32, FP64)
     int prof_n[2] = {0, 0};              // passes launched per phase
-    int chunk64() const { return precision == PASC_FP32_BULK ? chunk / 2 : chunk; }
 };
 
 struct Ctx {
@@ -149,7 +155,7 @@ struct Ctx {
     bool own_stream = false;
     int n_cu = 0;
     std::string arch;
-    // autotune cache: key -> launch shape
+    // autotune cache: key -> launch shapes of the two phases
     std::vector<std::pair<std::string, pasc_launch_shape>> tune;
 };
 
@@ -176,8 +182,9 @@ int engine_labels(Engine &e, int inst, int32_t *labels);
 int engine_abserr(Engine &e, int inst, const double *solution, double *abserr);
 int engine_adjoint(Engine &e, int inst, const uint8_t *mask, double cg_tol, int cg_maxit,
                    double *loss, double *dlogeps, double *dtheta, int *cg_it);
-int engine_autotune(Engine &e, int *block, int *chunk, int *prefetch, int *max_slots, double *us);
-int engine_set_shape(Engine &e, int block, int chunk, int prefetch, int max_slots);
+// us[0] / us[1]: measured pass time of the chosen FP32 / FP64 shape (0 when not measured)
+int engine_autotune(Engine &e, PassShape *s32, PassShape *s64, double us[2]);
+int engine_set_shape(Engine &e, const PassShape &s32, const PassShape &s64);
 
 int device_copy_bandwidth(Ctx *ctx, int64_t bytes, int reps, double *gbps);
 

@@ -196,15 +196,22 @@ class Problem:
     def set_profiling(self, enable: bool = True):
         check(lib().pasc_set_profiling(self.h, int(enable)))
 
-    def autotune(self, cache_path: str | None = None):
-        """Returns (block, chunk, prefetch, us_per_pass, max_slots) of the chosen launch shape."""
+    def autotune(self, cache_path: str | None = None) -> dict:
+        """Tunes the launch shapes; returns {"main": {...}, "polish": {...}} (see LaunchShape)."""
         sh = capi.LaunchShape()
         check(lib().pasc_autotune(self.h, cache_path.encode() if cache_path else None,
                                   C.byref(sh)))
-        return sh.block, sh.chunk, sh.prefetch, sh.us_per_pass, sh.max_slots
+        return {"main": sh.main.as_dict(), "polish": sh.polish.as_dict()}
 
-    def set_launch_shape(self, block: int, chunk: int, prefetch: int = 0, max_slots: int = 0):
-        sh = capi.LaunchShape(block, chunk, prefetch, max_slots, 0.0)
+    def set_launch_shape(self, block: int, chunk: int, prefetch: int = 0, max_slots: int = 0,
+                         min_tiles: int = 0, polish=None):
+        """main shape (FP64 problems: every pass; FP32-bulk: the FP32 phase); polish: a dict
+        for the FP64 polish of FP32-bulk problems (default: same shape, chunk / 2)."""
+        sh = capi.LaunchShape()
+        sh.main = capi.PassShape(block, chunk, prefetch, max_slots, min_tiles, 0.0)
+        if polish:
+            sh.polish = capi.PassShape(polish["block"], polish["chunk"], polish.get("prefetch", 0),
+                                       polish.get("max_slots", 0), polish.get("min_tiles", 0), 0.0)
         check(lib().pasc_set_launch_shape(self.h, C.byref(sh)))
 
 
@@ -252,6 +259,12 @@ class Batch:
         g = np.empty(self.xs[s].size * self.Ks[s])
         check(lib().pasc_batch_get_gamma(self.h, s, e, dptr(g), layout))
         return g
+
+    def autotune(self, cache_path: str | None = None) -> dict:
+        sh = capi.LaunchShape()
+        check(lib().pasc_batch_autotune(self.h, cache_path.encode() if cache_path else None,
+                                        C.byref(sh)))
+        return {"main": sh.main.as_dict(), "polish": sh.polish.as_dict()}
 
 
 class ProblemBatch:

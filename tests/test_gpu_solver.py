@@ -231,11 +231,17 @@ def test_autotune_keeps_iterate_and_results(ctx, oracle):
         P.set_theta(theta)
         g0 = oracle.project_simplex(np.random.default_rng(1).random(T * K), K)
         P.set_gamma(g0)
-        block, chunk, pf, us, _ = P.autotune()
-        print(f"prec={prec}: block={block} chunk={chunk} prefetch={pf} {us:.1f} us/pass")
-        assert block in (128, 256, 512) and pf in (0, 1) and us > 0
+        sh = P.autotune()
+        print(f"prec={prec}: {sh}")
+        m = sh["main"]
+        assert m["block"] in (128, 256, 512) and m["prefetch"] in (0, 1) and m["us_per_pass"] > 0
+        if prec == capi.FP32_BULK:
+            assert m["chunk"] in (4, 8) and sh["polish"]["chunk"] in (2, 4)
+            assert sh["polish"]["us_per_pass"] > 0
         np.testing.assert_allclose(P.get_gamma(), g0, atol=1e-7 if prec else 0)
-        assert P.autotune()[:3] == (block, chunk, pf)  # cached per context
+        sh2 = P.autotune()  # cached per context
+        assert {k: v for k, v in sh2["main"].items() if k != "us_per_pass"} == \
+            {k: v for k, v in m.items() if k != "us_per_pass"}
         P.set_gamma_uniform()
         r = P.gamma_solve(maxit=100000, stop_mask=capi.STOP_KKT, kkt_tol=1e-9)
         g = P.get_gamma()
@@ -245,9 +251,9 @@ def test_autotune_keeps_iterate_and_results(ctx, oracle):
 @pytest.mark.parametrize("prec_name", ["FP64", "FP32_BULK"])
 def test_launch_shapes_give_identical_iterates(ctx, prec_name):
     """Prefetching the next tile changes only the load order: with the same block, chunk and
-    slot count the iterates are bit-identical.  max_slots = 3 makes every workgroup run many
-    consecutive tiles, so the pipelined loads and the interior / boundary tile switch are
-    exercised at a small T.  (Other slot counts change the order of the FP64 partial sums,
+    slot tiling the iterates are bit-identical.  max_slots = 3 (or min_tiles = 4) makes every
+    workgroup run several consecutive tiles, so the pipelined loads and the interior / boundary
+    tile switch are exercised at a small T.  (Other slot counts change the order of the FP64 partial sums,
     which the BB steps amplify along the trajectory: those runs are not compared mid-solve.)"""
     from pasc_inference_amd import capi
     from pasc_inference_amd.solver import Problem
@@ -260,16 +266,47 @@ def test_launch_shapes_give_identical_iterates(ctx, prec_name):
     P.set_theta(theta)
     for block in (128, 256):
         for chunk in chunks:
-            outs = []
+            tilings = [(3, 0), (0, 0), (0, 4)]  # (max_slots, min_tiles)
+            outs = {}
             for pf in (0, 1):
-                for slots in (3, 0):
-                    P.set_launch_shape(block, chunk, pf, slots)
+                for slots, mt in tilings:
+                    P.set_launch_shape(block, chunk, pf, slots, mt)
                     P.set_gamma_uniform()
                     r = P.gamma_solve(maxit=300, stop_mask=0, polish_maxit=50)
-                    outs.append((pf, slots, r.it, r.fx, P.get_gamma()))
-            print(block, chunk, [(o[0], o[1], o[2], o[3]) for o in outs])
-            # same slot count: prefetch on and off agree bit for bit
-            assert outs[0][2] == outs[2][2] and np.array_equal(outs[0][4], outs[2][4])
-            assert outs[1][2] == outs[3][2] and np.array_equal(outs[1][4], outs[3][4])
+                    outs[(pf, slots, mt)] = (r.it, r.fx, P.get_gamma())
+            print(block, chunk, {k: v[:2] for k, v in outs.items()})
+            # same tiling: prefetch on and off agree bit for bit
+            for slots, mt in tilings:
+                a, b = outs[(0, slots, mt)], outs[(1, slots, mt)]
+                assert a[0] == b[0] and np.array_equal(a[2], b[2]), (slots, mt)
     with pytest.raises(capi.PascError):
         P.set_launch_shape(192, chunks[0])
+
+
+def test_batch_autotune_keeps_results(ctx):
+    """The batch autotuner restores gamma0 = 1/K and the tuned batch solves to the same
+    minimisers as the untuned one."""
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Batch
+    rng = np.random.default_rng(6)
+    xs, Ks, ths = [], [], []
+    for i, T in enumerate([3000, 7000, 12000]):
+        K = 2 + i
+        levels = np.linspace(0.0, 3.0, K)
+        x, _ = _signal(40 + i, T, levels, 2e-3, 0.5)
+        xs.append(x)
+        Ks.append(K)
+        ths.append(levels)
+    eps = np.array([1.0, 10.0])
+    params = dict(maxit=100000, stop_mask=capi.STOP_KKT, kkt_tol=1e-9)
+    B0 = Batch(ctx, xs, Ks, ths, eps, precision=capi.FP32_BULK)
+    r0, _ = B0.solve(**params)
+    B1 = Batch(ctx, xs, Ks, ths, eps, precision=capi.FP32_BULK)
+    sh = B1.autotune()
+    print(sh)
+    r1, _ = B1.solve(**params)
+    for s in range(3):
+        for e in range(2):
+            j = s * 2 + e
+            assert r0[j].kkt <= 1e-9 and r1[j].kkt <= 1e-9
+            assert np.max(np.abs(B0.get_gamma(s, e) - B1.get_gamma(s, e))) <= 1e-6

@@ -256,4 +256,22 @@ __device__ __forceinline__ P *uni(P *v) {
     return reinterpret_cast<P *>(uni((int64_t)reinterpret_cast<uintptr_t>(v)));
 }
 
+// Hand-over of a few values between workgroups within one launch, without an agent-scope fence
+// (on gfx950 a release fence writes back every dirty line of the XCD's L2).  The values are
+// agent-scope atomic stores, which write through to the device coherence point; publish_wait()
+// waits until they completed, after which the writer counts them with an atomic RMW; the
+// reader, once it observed the count, reads them with agent-scope atomic loads, which read from
+// the coherence point rather than a possibly stale line of its own XCD's L2.
+__device__ __forceinline__ void store_coherent(double *p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double load_coherent(const double *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void publish_wait() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);  // keeps the compiler from moving the stores
+    __builtin_amdgcn_s_waitcnt(0);            // all vector memory operations completed
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 }  // namespace pasc

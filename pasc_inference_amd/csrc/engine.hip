@@ -153,6 +153,8 @@ int engine_alloc(Engine &e, Ctx *ctx, int precision, int stencil, int32_t grid_w
     HIP_TRY(hipMalloc(&e.d_desc, sizeof(InstDesc) * std::max(e.n_inst, 1)));
     HIP_TRY(hipMalloc(&e.d_state, sizeof(InstState) * std::max(e.n_inst, 1)));
     HIP_TRY(hipMalloc(&e.d_nactive, 2 * sizeof(int32_t)));  // [0] active count, [1] dirty flag
+    HIP_TRY(hipMalloc(&e.d_done, sizeof(int32_t) * DONE_WORDS));
+    HIP_TRY(hipMemsetAsync(e.d_done, 0, sizeof(int32_t) * DONE_WORDS, S_(e)));
     // pinned poll area: [0] active instances, [1 + g] active tiles of K group g
     HIP_TRY(hipHostMalloc((void **)&e.h_nactive, sizeof(int32_t) * (1 + PASC_MAX_K),
                           hipHostMallocDefault));
@@ -197,6 +199,8 @@ void engine_free(Engine &e) {
     if (e.d_state) (void)hipFree(e.d_state);
     if (e.d_part) (void)hipFree(e.d_part);
     if (e.d_nactive) (void)hipFree(e.d_nactive);
+    if (e.d_done) (void)hipFree(e.d_done);
+    e.d_done = nullptr;
     if (e.h_nactive) (void)hipHostFree(e.h_nactive);
     if (e.d_scratch) (void)hipFree(e.d_scratch);
     for (auto &g : e.groups) {
@@ -373,6 +377,7 @@ struct PhaseCfg {
 static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<int> &grids,
                              std::vector<hipEvent_t> *ev = nullptr) {
     hipStream_t s = S_(e);
+    const bool fused = e.n_inst == 1 && e.stencil == STENCIL_TIME;
     if (ev) {
         hipEvent_t a;
         HIP_TRY(hipEventCreate(&a));
@@ -389,7 +394,8 @@ static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<in
         if (e.stencil == STENCIL_TIME) {
             const void *bb = ph.f64 ? e.d_b64 : e.d_b32;
             PassArgs a{g0, g1, x, e.d_desc, e.d_state, e.d_part, gv, ph.kkt_every,
-                       ph.sp.fx_identity ? 0 : 1, bb};
+                       ph.sp.fx_identity ? 0 : 1, bb, ph.sp, fused ? e.d_done : nullptr,
+                       e.d_nactive, e.d_nactive + 1};
             const PassShape &sh = e.sh[ph.f64 ? 1 : 0];
             rc = launch_pass_1d(ph.f64, g.K, a, grids[gi], sh.block, sh.chunk, sh.prefetch, s);
         } else {
@@ -406,6 +412,12 @@ static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<in
         HIP_TRY(hipEventCreate(&b));
         HIP_TRY(hipEventRecord(b, s));
         ev->push_back(b);
+    }
+    // A single 1-D problem: the pass ran the step (last workgroup) and its tiles never need
+    // compacting (the slots of a stopped instance are skipped), so nothing else is launched.
+    if (fused) {
+        HIP_TRY(hipGetLastError());
+        return PASC_OK;
     }
     hipLaunchKernelGGL(spg_step, dim3(std::min(e.n_inst, std::max(e.ctx->n_cu, 1) * 4)), dim3(256),
                        0, s, e.d_desc, e.d_state, e.d_part, e.n_inst, ph.sp, e.d_nactive + 1);

@@ -17,6 +17,7 @@
 #pragma once
 
 #include "device_common.h"
+#include "step_scalar.h"
 
 namespace pasc {
 
@@ -25,12 +26,19 @@ struct PassArgs {
     void *gamma1;
     const void *x;
     const InstDesc *desc;
-    const InstState *state;
+    InstState *state;
     double *part;
     GroupView grp;
     int32_t kkt_every;  // evaluate the KKT residual on passes with it % kkt_every == 0
     int32_t need_f;     // 0: f comes from the quadratic identity (only the first pass sums it)
     const void *b;      // explicit linear terms of the phase's type, indexed like gamma (or null)
+    // fused step (single problems): the workgroup that completes the instance's last partial
+    // slot reduces the slots and runs the SPG step (no separate step launch)
+    StepParams sp;
+    int32_t *done;      // DONE_WORDS counters of completed slots in this pass (back to 0
+                        // after); null: the step kernel runs the step
+    int32_t *nactive;   // active instances (decremented when one stops)
+    int32_t *dirty;     // set when an instance stops (the batch's tile scan rebuilds)
 };
 
 template <int CH>
@@ -414,6 +422,18 @@ __device__ __forceinline__ void pass_tile_1d(const TileCtx<S, C, K> &cx, int chu
     }
 }
 
+// The step of one instance, run by the workgroup that completed its last partial slot: the
+// slots are reduced in a fixed order and step_scalar advances the instance.
+template <int NT>
+__device__ __forceinline__ void fused_step(const PassArgs &A, int inst, InstState *lds_state,
+                                           double *lds_red) {
+    const InstDesc &D = A.desc[inst];
+    const double *part = A.part + (int64_t)uni(D.tile0) * NPART;
+    step_instance<NT>(A.state + inst, uni(D.ntiles), D.alpha * D.s_kkt, A.sp, A.dirty, lds_state,
+                      lds_red, [&](int i, int q) { return load_coherent(part + (int64_t)i * NPART + q); });
+    if (threadIdx.x == 0 && !lds_state->active) atomicSub(A.nactive, 1);
+}
+
 // PF: software pipelining across the msub tiles of a slot.  While a tile is computed, the
 // next interior tile's rows are copied into an LDS stage by the asynchronous global -> LDS path
 // (no registers held for it); the wait for those copies is the only added synchronisation.
@@ -427,13 +447,19 @@ __global__ __launch_bounds__(NT) void spg_pass_1d(PassArgs A) {
     __shared__ double lds_red[NPART * (NT / WAVE)];
     __shared__ C lds_x[2 * K * NT];
     __shared__ __attribute__((aligned(16))) S stage[USE_STAGE ? SG::ROWS * SG::ROW : 1];
+    __shared__ int32_t finished;  // instance whose last slot this workgroup wrote, or -1
+    __shared__ InstState lds_state;  // fused step
     constexpr int TILE = tile_steps<NT, CH>();
+    if (threadIdx.x == 0) finished = -1;
     const int total = A.grp.off[A.grp.n];
     for (int j = blockIdx.x; j < total; j += gridDim.x) {
         int inst, slot;
         find_tile(A.grp, j, inst, slot);
         inst = uni(inst);
         slot = uni(slot);
+        // an instance that stopped keeps its tiles until the next scan rebuild (batches) or for
+        // good (single problems): its slots are skipped
+        if (uni(A.state[inst].active) == 0) continue;
         TileCtx<S, C, K> cx;
         make_tile_ctx<S, C, K>(A, A.desc[inst], A.state[inst], cx);
         const int msub = uni(A.desc[inst].msub);
@@ -472,9 +498,41 @@ __global__ __launch_bounds__(NT) void spg_pass_1d(PassArgs A) {
         block_reduce_part<NT>(acc, lds_red);
         if (threadIdx.x == 0) {
             double *o = A.part + (int64_t)slot_out * NPART;
+            if (A.done) {
+                // Single problems: publish the slot, then count it; the workgroup that
+                // completes the instance runs its step after its last tile (the reduction
+                // order over the slots is fixed, so the result does not depend on which
+                // workgroup that is).  The slot is written through to the device coherence
+                // point and the count is taken once those stores completed; an agent-scope
+                // release fence would also write back every dirty line of this XCD's L2 (the
+                // gamma rows in flight), which costs a sixth of the pass.
 #pragma unroll
-            for (int i = 0; i < NPART; i++) o[i] = acc[i];
+                for (int i = 0; i < NPART; i++) store_coherent(o + i, acc[i]);
+                publish_wait();
+                const int ns = uni(A.desc[inst].ntiles);
+                const int g = slot % DONE_SUB;
+                const int n_g = ns / DONE_SUB + (g < ns % DONE_SUB ? 1 : 0);
+                int32_t *sub = A.done + g * DONE_STRIDE;
+                if (atomicAdd(sub, 1) == n_g - 1) {
+                    __hip_atomic_store(sub, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    int32_t *top = A.done + DONE_SUB * DONE_STRIDE;
+                    const int n_top = ns < DONE_SUB ? ns : DONE_SUB;
+                    if (atomicAdd(top, 1) == n_top - 1) {
+                        __hip_atomic_store(top, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        finished = inst;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < NPART; i++) o[i] = acc[i];
+            }
         }
+    }
+    // after the tile loop, so that the step's registers do not add to the loop's
+    if (A.done) {
+        __syncthreads();
+        const int fin = finished;
+        if (fin >= 0) fused_step<NT>(A, fin, &lds_state, lds_red);
     }
 }
 

@@ -968,7 +968,8 @@ static int autotune_cached(Engine &e, Ctx &c, const std::string &key, int64_t si
                 if (!strstr(line, want)) continue;
                 const char *ps = strstr(line, "\"shape\": [");
                 int v[10];
-                if (ps && sscanf(ps + 10, "[%d, %d, %d, %d, %d, %d, %d, %d, %d, %d]", &v[0], &v[1],
+                // ps + 9 is the '[' of the list
+                if (ps && sscanf(ps + 9, "[%d, %d, %d, %d, %d, %d, %d, %d, %d, %d]", &v[0], &v[1],
                                  &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9]) == 10) {
                     sh.main = pasc_pass_shape{v[0], v[1], v[2], v[3], v[4], 0.0};
                     sh.polish = pasc_pass_shape{v[5], v[6], v[7], v[8], v[9], 0.0};

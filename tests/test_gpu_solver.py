@@ -283,6 +283,29 @@ def test_launch_shapes_give_identical_iterates(ctx, prec_name):
         P.set_launch_shape(192, chunks[0])
 
 
+def test_autotune_file_cache(tmp_path):
+    """A second context finds the shape the first one tuned in the cache file (no re-tuning)."""
+    import json
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Context, Problem
+    T, K = 20000, 3
+    x, _ = _signal(23, T, [0.0, 1.0, 3.0], 1e-3, 0.9)
+    path = str(tmp_path / "tune.jsonl")
+    shapes = []
+    for _ in range(2):
+        with Context(0) as c:
+            P = Problem(c, x, K, precision=capi.FP64, epssqr=5.0)
+            P.set_theta(np.array([0.0, 1.0, 3.0]))
+            P.set_gamma_uniform()
+            shapes.append(P.autotune(path))
+            P.close()
+    lines = [json.loads(l) for l in open(path)]
+    assert [l["cached"] for l in lines] == [0, 1]
+    assert lines[0]["shape"] == lines[1]["shape"]
+    drop = lambda d: {k: v for k, v in d.items() if k != "us_per_pass"}
+    assert drop(shapes[0]["main"]) == drop(shapes[1]["main"])
+
+
 def test_batch_autotune_keeps_results(ctx):
     """The batch autotuner restores gamma0 = 1/K and the tuned batch solves to the same
     minimisers as the untuned one."""

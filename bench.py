@@ -260,6 +260,10 @@ def run_cpu_baseline(args, c2):
         out[nth] = dt / args.cpu_iters
     it_needed = c2["r64"].it
     best = min(out, key=lambda k: out[k])
+    # the same algorithmic-byte convention as the device (6 n s per iteration, s = 8) against a
+    # host copy of 2 x 512 MiB (bytes read + written per second)
+    alg_bytes = 6.0 * T * K * 8
+    stream = {str(nth): pyoracle.host_copy_bandwidth(1 << 29, 3, nth) / 1e9 for nth in sorted({1, cores})}
     return dict(
         value=out[best] * it_needed, unit="s", cores=best, kind="port",
         sample=(f"{args.cpu_iters} op-by-op FP64 SPG iterations of the T={T} K={K} solve on "
@@ -267,6 +271,8 @@ def run_cpu_baseline(args, c2):
                 f"{out[1] * 1e3:.1f} ms/iteration) x {it_needed} iterations to KKT "
                 f"{args.kkt:g} (the FP64 device path's count for the same algorithm)"),
         ms_per_iteration={str(k): v * 1e3 for k, v in out.items()},
+        algorithmic_gbps={str(k): alg_bytes / v / 1e9 for k, v in out.items()},
+        host_copy_gbps=stream,
     )
 
 

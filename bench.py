@@ -168,7 +168,10 @@ def _roofline(kernel, s, n, T, avg, traffic, copy_gbps):
     """The dominant kernel against the HBM roofline: achieved = algorithmic bytes per launch
     (6*n*s, the vectors the reference iteration moves) / average launch time."""
     alg = 6.0 * n * s
-    own = (2.0 * n + T) * s  # gamma_{i-1} read, gamma_i written, data read
+    # K - 1 rows of gamma_{i-1} read and of gamma_i written (the last row is 1 - the others),
+    # data read
+    K = n // T
+    own = (2.0 * (K - 1 if K >= 2 else K) * T + T) * s
     out = {"bound": "hbm", "achieved": alg / avg / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": alg / avg / 1e9 / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
            "avg_launch_us": avg * 1e6, "algorithmic_bytes_per_launch": alg,

@@ -30,6 +30,22 @@ int or_max_threads(void) {
 
 static int fix_threads(int n) { return n < 1 ? 1 : n; }
 
+/* Sums over many threads are formed as one partial per thread (static schedule) added in
+ * thread order, so that a result does not depend on the order in which threads finish (an
+ * OpenMP reduction clause leaves that order open, which made SPG trajectories vary run to
+ * run). */
+#define OR_MAX_THREADS 256
+#define DET_SUM_BEGIN(nthreads)                                                              \
+    double part_[OR_MAX_THREADS] = {0.0};                                                    \
+    if ((nthreads) > OR_MAX_THREADS) (nthreads) = OR_MAX_THREADS;                           \
+    _Pragma("omp parallel num_threads(nthreads)") {                                          \
+        double s_ = 0.0;
+#define DET_SUM_END(result)                                                                  \
+        part_[omp_get_thread_num()] = s_;                                                    \
+    }                                                                                        \
+    (result) = 0.0;                                                                          \
+    for (int i_ = 0; i_ < OR_MAX_THREADS; i_++) (result) += part_[i_];
+
 /* ------------------------------------------------------------------------------------------ */
 /* simplex projection                                                                         */
 /* ------------------------------------------------------------------------------------------ */
@@ -178,20 +194,28 @@ void or_theta_update(const double *x, const double *gamma, double *theta, int64_
     nthreads = fix_threads(nthreads);
     const int64_t nodes = T * R;
     for (int k = 0; k < K; k++) {
-        double gsum = 0.0;
+        double gsum;
         /* VecSum(gamma_k), graphh1fem.cpp:364 */
-#pragma omp parallel for schedule(static) num_threads(nthreads) reduction(+ : gsum)
-        for (int64_t i = 0; i < nodes; i++) {
-            if (heldout && heldout[i]) continue;
-            gsum += gamma[i * K + k];
-        }
-        for (int c = 0; c < xdim; c++) {
-            double gx = 0.0;
-            /* VecDot(data, gamma_k), graphh1fem.cpp:361 */
-#pragma omp parallel for schedule(static) num_threads(nthreads) reduction(+ : gx)
+        {
+            DET_SUM_BEGIN(nthreads)
+#pragma omp for schedule(static)
             for (int64_t i = 0; i < nodes; i++) {
                 if (heldout && heldout[i]) continue;
-                gx += x[i * xdim + c] * gamma[i * K + k];
+                s_ += gamma[i * K + k];
+            }
+            DET_SUM_END(gsum)
+        }
+        for (int c = 0; c < xdim; c++) {
+            double gx;
+            /* VecDot(data, gamma_k), graphh1fem.cpp:361 */
+            {
+                DET_SUM_BEGIN(nthreads)
+#pragma omp for schedule(static)
+                for (int64_t i = 0; i < nodes; i++) {
+                    if (heldout && heldout[i]) continue;
+                    s_ += x[i * xdim + c] * gamma[i * K + k];
+                }
+                DET_SUM_END(gx)
             }
             /* graphh1fem.cpp:375-379 */
             theta[k * xdim + c] = (gsum != 0) ? gx / gsum : 0.0;
@@ -244,21 +268,89 @@ static void v_axpy(double *y, double a, const double *x, int64_t n, int nthreads
 }
 
 static double v_dot(const double *a, const double *b, int64_t n, int nthreads) {
-    double s = 0.0;
-#pragma omp parallel for schedule(static) num_threads(nthreads) reduction(+ : s)
-    for (int64_t i = 0; i < n; i++) s += a[i] * b[i];
+    double s;
+    DET_SUM_BEGIN(nthreads)
+#pragma omp for schedule(static)
+    for (int64_t i = 0; i < n; i++) s_ += a[i] * b[i];
+    DET_SUM_END(s)
     return s;
 }
 
 /* sum_t sum_k (g_kt - g_0t) d_kt: the value of dot(g, d) when every block of d sums to 0 */
 static double dot_centered(const double *d, const double *g, int64_t nblocks, int K,
                            int nthreads) {
-    double s = 0.0;
-#pragma omp parallel for schedule(static) num_threads(nthreads) reduction(+ : s)
+    double s;
+    DET_SUM_BEGIN(nthreads)
+#pragma omp for schedule(static)
     for (int64_t t = 0; t < nblocks; t++) {
         const double c = g[t * K];
-        for (int k = 1; k < K; k++) s += (g[t * K + k] - c) * d[t * K + k];
+        for (int k = 1; k < K; k++) s_ += (g[t * K + k] - c) * d[t * K + k];
     }
+    DET_SUM_END(s)
+    return s;
+}
+
+/* gd of the projected-gradient direction in a form whose sign survives rounding (the
+ * gd_certified option): d = P(y) - x with y = x - a g, so g = (x - y)/a and
+ * <g, d> = -(|d|^2 + sum_{k: z_k = 0} (tau - y_k) x_k) / a, tau the threshold of P(y). */
+static double dot_certified(const double *x, const double *g, const double *d, int64_t nblocks,
+                            int K, double a, int nthreads) {
+    double s;
+    DET_SUM_BEGIN(nthreads)
+    double *y = (double *)malloc(sizeof(double) * (size_t)K * 3);
+    double *z = y + K, *w = y + 2 * K;
+#pragma omp for schedule(static)
+    for (int64_t t = 0; t < nblocks; t++) {
+        int m = 0;
+        for (int k = 0; k < K; k++) {
+            y[k] = x[t * K + k] - a * g[t * K + k];
+            z[k] = y[k];
+        }
+        project_block(z, K, w);
+        for (int k = 1; k < K; k++)
+            if (z[k] > z[m]) m = k;
+        const double tau = y[m] - z[m];
+        double o = 0.0;
+        for (int k = 0; k < K; k++) {
+            const double dk = d[t * K + k];
+            o += dk * dk;
+            if (y[k] <= tau) o += (tau - y[k]) * x[t * K + k];
+        }
+        s_ += o;
+    }
+    free(y);
+    DET_SUM_END(s)
+    return -s / a;
+}
+
+/* d'A0 d as a sum over the edges of A0 (the dad_edges option): each node adds half of
+ * w (d_i - d_j)^2 over its neighbours j, with the weights of or_h1_matvec's rows. */
+static double dad_by_edges(const double *d, int64_t T, int64_t R, int K, const int64_t *nbr_ptr,
+                           const int32_t *nbr_idx, int nthreads) {
+    const int64_t RK = R * K;
+    double s;
+    DET_SUM_BEGIN(nthreads)
+#pragma omp for schedule(static)
+    for (int64_t t = 0; t < T; t++) {
+        for (int64_t r = 0; r < R; r++) {
+            const int64_t beg = nbr_ptr ? nbr_ptr[r] : 0;
+            const int64_t end = nbr_ptr ? nbr_ptr[r + 1] : 0;
+            for (int k = 0; k < K; k++) {
+                const int64_t i = t * RK + r * K + k;
+                double acc = 0.0, e;
+                if (t > 0) { e = d[i] - d[i - RK]; acc += 2.0 * e * e; }
+                if (t < T - 1) { e = d[i] - d[i + RK]; acc += 2.0 * e * e; }
+                for (int64_t q = beg; q < end; q++) {
+                    const int64_t i2 = t * RK + (int64_t)nbr_idx[q] * K + k;
+                    e = d[i] - d[i2]; acc += e * e;
+                    if (t > 0) { e = d[i] - d[i2 - RK]; acc += e * e; }
+                    if (t < T - 1) { e = d[i] - d[i2 + RK]; acc += e * e; }
+                }
+                s_ += 0.5 * acc;
+            }
+        }
+    }
+    DET_SUM_END(s)
     return s;
 }
 
@@ -360,8 +452,12 @@ int or_spg_solve(double *x, const double *b, int64_t T, int64_t R, int K, double
 
         /* compute_dots :348-367 */
         dd = v_dot(d, d, n, nthreads);
-        dAd = v_dot(d, Ad, n, nthreads);
-        gd = p->gd_centered ? dot_centered(d, g, nblocks, K, nthreads) : v_dot(d, g, n, nthreads);
+        dAd = p->dad_edges ? alpha * dad_by_edges(d, T, R, K, nbr_ptr, nbr_idx, nthreads)
+                           : v_dot(d, Ad, n, nthreads);
+        if (p->gd_certified)
+            gd = dot_certified(x, g, d, nblocks, K, alpha_bb, nthreads);
+        else
+            gd = p->gd_centered ? dot_centered(d, g, nblocks, K, nthreads) : v_dot(d, g, n, nthreads);
 
         fx_max = fs_get_max(&fs); /* :176 */
 
@@ -381,7 +477,12 @@ int or_spg_solve(double *x, const double *b, int64_t T, int64_t R, int K, double
 
         /* x += beta*d; g += beta*Ad (:200-201) */
         v_axpy(x, beta, d, n, nthreads);
-        v_axpy(g, beta, Ad, n, nthreads);
+        if (p->g_recompute) {
+            or_h1_matvec(x, g, T, R, K, alpha, nbr_ptr, nbr_idx, nthreads);
+            v_axpy(g, -1.0, b, n, nthreads);
+        } else {
+            v_axpy(g, beta, Ad, n, nthreads);
+        }
 
         /* :207-211 */
         fx_old = fx;

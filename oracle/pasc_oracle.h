@@ -81,6 +81,16 @@ typedef struct {
                              since sum_k d_kt = 0; removes the common-mode rounding) */
     int safeguard;        /* dAd <= 0 (d in the null space of A): beta = sigma2 and the BB step
                              restarts at alphainit, where the reference divides by zero */
+    int g_recompute;      /* g = A x - b from x after every update instead of g += beta*Ad
+                             (:200-201): no drift of g over long runs */
+    int gd_certified;     /* gd = -(dd + sum over the off-support entries of (tau - y_k) x_k)
+                             / alpha_bb, with y = x - alpha_bb g and tau the projection's
+                             threshold: equal in exact arithmetic (d = P(y) - x, g = (x - y) /
+                             alpha_bb), and never positive, so its sign survives rounding */
+    int dad_edges;        /* dAd = alpha * sum over the edges of A0 of w (d_i - d_j)^2 instead of
+                             dot(d, A d): non-negative terms, no cancellation.  These three keep
+                             the SPG converging to KKT 1e-12; with the op-by-op formulas the
+                             residual can stall near 1e-9 once d is at rounding level */
 } or_spg_params;
 
 typedef struct {

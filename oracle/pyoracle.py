@@ -30,6 +30,7 @@ class SpgParams(C.Structure):
         ("stop_Anormgp", C.c_int), ("stop_Anormgp_normb", C.c_int), ("stop_kkt", C.c_int),
         ("kkt_tol", C.c_double), ("kkt_scale", C.c_double), ("nthreads", C.c_int),
         ("fx_identity", C.c_int), ("gd_centered", C.c_int), ("safeguard", C.c_int),
+        ("g_recompute", C.c_int), ("gd_certified", C.c_int), ("dad_edges", C.c_int),
     ]
 
 

@@ -513,6 +513,22 @@ static int run_phase(Engine &e, const PhaseCfg &ph, const pasc_spg_params &p, in
     }
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (e.stencil == STENCIL_TIME) {
+        // the passes stored K - 1 rows: write the last one of every current iterate
+        int64_t tmax = 1;
+        for (int i = 0; i < e.n_inst; i++) tmax = std::max<int64_t>(tmax, e.h_desc[i].T);
+        const dim3 grid((unsigned)std::min<int64_t>((tmax + 255) / 256, 2048),
+                        (unsigned)std::min(std::max(e.n_inst, 1), 65535));
+        const bool g64 = ph.f64 && e.precision == PASC_FP32_BULK;
+        if (ph.f64)
+            hipLaunchKernelGGL(complete_last_row<double>, grid, dim3(256), 0, s, e.d_desc, e.d_state,
+                               e.n_inst, (double *)(g64 ? e.d_gamma64[0] : e.d_gamma[0]),
+                               (double *)(g64 ? e.d_gamma64[1] : e.d_gamma[1]));
+        else
+            hipLaunchKernelGGL(complete_last_row<float>, grid, dim3(256), 0, s, e.d_desc, e.d_state,
+                               e.n_inst, (float *)e.d_gamma[0], (float *)e.d_gamma[1]);
+        HIP_TRY(hipGetLastError());
+    }
     if (!evs.empty()) {
         HIP_TRY(hipStreamSynchronize(s));
         double ms = 0.0;
@@ -532,7 +548,11 @@ static void init_states(Engine &e, std::vector<InstState> &st, const pasc_spg_pa
                         const std::vector<char> &mask, bool keep_alpha) {
     for (int i = 0; i < e.n_inst; i++) {
         InstState &s = st[i];
-        if (!mask[i]) { s.active = 0; continue; }
+        if (!mask[i]) {
+            s.active = 0;
+            s.first = 1;  // runs no pass: its rows are left as they are
+            continue;
+        }
         const double a = keep_alpha ? s.alpha_dir : p.alphainit;
         const double nb = s.normb;
         std::memset(&s, 0, sizeof(s));

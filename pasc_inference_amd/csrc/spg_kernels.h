@@ -44,6 +44,14 @@ struct PassArgs {
 template <int CH>
 constexpr int halo_threads() { return (3 + CH - 1) / CH; }
 
+// Rows of gamma the 1-D pass reads and writes.  Every iterate after the first pass of a phase
+// lies on the simplex, so its last row is 1 - (sum of the others): the pass stores K - 1 rows
+// and rebuilds the last one in registers (clamped at 0, so that it stays exactly feasible; the
+// sum is then 1 up to rounding, as for the projection's own output).  complete_last_row writes
+// the full row once a phase ends, so that every other reader sees K rows.
+template <int K>
+constexpr int stored_rows() { return K >= 2 ? K - 1 : K; }
+
 template <int NT, int CH>
 constexpr int tile_steps() { return (NT - 2 * halo_threads<CH>()) * CH; }
 
@@ -170,7 +178,7 @@ template <typename S, int K, int CH, int NT>
 struct Stage {
     static constexpr int ROW = NT * CH;             // elements per staged row
     static constexpr int Q = CH * (int)sizeof(S) / 16;  // 16-byte copies per lane and row
-    static constexpr int ROWS = K + 1;              // K gamma rows + the data row
+    static constexpr int ROWS = stored_rows<K>() + 1;  // the stored gamma rows + the data row
     static constexpr size_t BYTES = (size_t)ROWS * ROW * sizeof(S);
     static_assert(Q >= 1 && CH * sizeof(S) % 16 == 0, "a thread's chunk must be whole 16-byte pieces");
 };
@@ -184,11 +192,12 @@ __device__ __forceinline__ void stage_issue(const TileCtx<S, C, K> &c, int tile,
     constexpr int HT = halo_threads<CH>();
     constexpr int TILE = tile_steps<NT, CH>();
     const int64_t lo = (int64_t)tile * TILE - (int64_t)HT * CH;  // >= 1 for an interior tile
+    constexpr int KS = stored_rows<K>();
     const int w = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
 #pragma unroll
-    for (int r = 0; r < K + 1; r++) {
-        const char *g = reinterpret_cast<const char *>(r < K ? c.src + (int64_t)r * c.ld + lo
-                                                             : c.xr + lo);
+    for (int r = 0; r < KS + 1; r++) {
+        const char *g = reinterpret_cast<const char *>(r < KS ? c.src + (int64_t)r * c.ld + lo
+                                                              : c.xr + lo);
         char *l = reinterpret_cast<char *>(stage + (size_t)r * SG::ROW);
 #pragma unroll
         for (int q = 0; q < SG::Q; q++) {
@@ -253,7 +262,7 @@ __device__ __forceinline__ void pass_tile_1d(const TileCtx<S, C, K> &cx, int chu
     } else {
         C X[CH];
         if (STAGED)
-            stage_read<S, C, K, CH, NT>(stage, K, X);
+            stage_read<S, C, K, CH, NT>(stage, stored_rows<K>(), X);
         else
             load_chunk<S, C, CH, INT>(cx.xr, t0, T, ld, X);
         bool obs[CH];
@@ -282,15 +291,32 @@ __device__ __forceinline__ void pass_tile_1d(const TileCtx<S, C, K> &cx, int chu
     const bool want_kkt = cx.want_kkt;
 
     // level 1: gamma_i = gamma_{i-1} + beta_{i-1} (P(gamma_{i-1} - alpha_{i-1} g_{i-1}) - gamma_{i-1})
+    constexpr int KS = stored_rows<K>();
     C G1[K][CH];
     {
         C G[K][CH];
 #pragma unroll
-        for (int k = 0; k < K; k++) {
+        for (int k = 0; k < KS; k++) {
             if (STAGED)
                 stage_read<S, C, K, CH, NT>(stage, k, G[k]);
             else
                 load_chunk<S, C, CH, INT>(src + k * ld, t0, T, ld, G[k]);
+        }
+        if (KS < K) {
+            if (first) {
+                // gamma_0 may lie off the simplex: its last row is read (first passes are
+                // never staged)
+                load_chunk<S, C, CH, INT>(src + (K - 1) * ld, t0, T, ld, G[K - 1]);
+            } else {
+#pragma unroll
+                for (int c = 0; c < CH; c++) {
+                    C r = C(1);
+#pragma unroll
+                    for (int k = 0; k < K - 1; k++) r -= G[k][c];
+                    const bool valid = INT || (t0 + c >= 0 && t0 + c < T);
+                    G[K - 1][c] = valid ? (r > C(0) ? r : C(0)) : C(0);
+                }
+            }
         }
         C gl[K], gr[K];
         exchange_edges<C, K, CH, NT>(G, lds_x, gl, gr);
@@ -391,7 +417,7 @@ __device__ __forceinline__ void pass_tile_1d(const TileCtx<S, C, K> &cx, int chu
             using VT = V16<S>;
             constexpr int VW = VT::n;
 #pragma unroll
-            for (int k = 0; k < K; k++) {
+            for (int k = 0; k < KS; k++) {
 #pragma unroll
                 for (int q = 0; q < CH / VW; q++) {
                     const int64_t t = t0 + q * VW;
@@ -468,7 +494,9 @@ __global__ __launch_bounds__(NT) void spg_pass_1d(PassArgs A) {
         double acc[NPART] = {0.0, 0.0, 0.0, 0.0, 0.0};
         const int first_tile = slot * msub;
         bool staged = false;  // the current tile's rows are in the stage
-        if (USE_STAGE && tile_is_interior<S, C, K, CH, NT>(first_tile, cx.T)) {
+        // a first pass reads all K rows of gamma_0 and runs unstaged
+        const bool can_stage = USE_STAGE && !cx.first;
+        if (can_stage && tile_is_interior<S, C, K, CH, NT>(first_tile, cx.T)) {
             stage_issue<S, C, K, CH, NT>(cx, first_tile, stage);
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
@@ -478,7 +506,7 @@ __global__ __launch_bounds__(NT) void spg_pass_1d(PassArgs A) {
             const int tile = first_tile + m;
             if ((int64_t)tile * TILE >= cx.T) break;
             const int nt = tile + 1;
-            const bool stage_next = USE_STAGE && m + 1 < msub && (int64_t)nt * TILE < cx.T &&
+            const bool stage_next = can_stage && m + 1 < msub && (int64_t)nt * TILE < cx.T &&
                                     tile_is_interior<S, C, K, CH, NT>(nt, cx.T);
             auto issue = [&]() {
                 if (stage_next) stage_issue<S, C, K, CH, NT>(cx, nt, stage);

@@ -247,4 +247,28 @@ __global__ __launch_bounds__(256) void convert_kernel(const A *src, B *dst, int6
     if (i < n) dst[i] = B(src[i]);
 }
 
+// ---- last gamma row of the 1-D engine --------------------------------------------------------
+// The fused 1-D pass stores K - 1 rows of each iterate (the last is 1 - the sum of the others,
+// clamped at 0).  At the end of a phase this writes the full last row of every instance's
+// current iterate, with the pass's own arithmetic, so that the row equals the value the pass
+// used.  Instances that ran no pass in the phase (first still set) keep their rows.  grid.y
+// strides over the instances, grid.x over the steps.
+template <typename S>
+__global__ __launch_bounds__(256) void complete_last_row(const InstDesc *desc,
+                                                         const InstState *state, int n_inst,
+                                                         S *g0, S *g1) {
+    for (int i = blockIdx.y; i < n_inst; i += gridDim.y) {
+        const InstDesc &D = desc[i];
+        const int K = D.K;
+        if (K < 2 || state[i].first) continue;
+        S *g = (state[i].parity ? g1 : g0) + D.goff;
+        for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < D.T;
+             t += (int64_t)gridDim.x * 256) {
+            S r = S(1);
+            for (int k = 0; k < K - 1; k++) r -= g[k * D.ld + t];
+            g[(K - 1) * D.ld + t] = r > S(0) ? r : S(0);
+        }
+    }
+}
+
 }  // namespace pasc

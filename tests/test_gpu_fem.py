@@ -59,7 +59,7 @@ def test_fem_alternation_matches_oracle(ctx, oracle, ftype, prec_name):
     x, truth = generate_markov(0, T, [1.0, 2.0], 0.002, 0.5)
     theta0 = np.array([0.8, 2.2])
     spg_o = dict(maxit=200000, stop_difff=0, stop_kkt=1, kkt_tol=1e-12, kkt_scale=T * r,
-                 nthreads=8, gd_centered=1)
+                 nthreads=8, gd_centered=1, gd_certified=1, dad_edges=1, g_recompute=1)
     ts = dict(maxit=5, eps=0.0, skip_first_theta=1, fem_reduce=r, fem_type=ftype)
     g_ref, th_ref, res_ref = oracle.ts_solve(x, np.full(T * K, 1.0 / K), theta0, T, 1, K, 1, eps,
                                              ts=ts, **spg_o)

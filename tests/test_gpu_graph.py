@@ -92,7 +92,7 @@ def test_spatiotemporal_solve_matches_oracle(ctx, oracle, prec_name, kind):
     b = oracle.assemble_b(x, theta, T, R, K)
     ref, rr = oracle.spg_solve(np.full(T * R * K, 1.0 / K), b, T, R, K, eps / T, graph=graph,
                                maxit=400000, stop_difff=0, stop_kkt=1, kkt_tol=1e-12, kkt_scale=T,
-                               nthreads=8, gd_centered=1)
+                               nthreads=8, gd_centered=1, gd_certified=1, dad_edges=1, g_recompute=1)
     assert rr.kkt <= 1e-12
     P.set_theta(theta)
     P.set_gamma_uniform()

@@ -44,7 +44,7 @@ def test_cpp_driver_epsilon_scan_on_reference_fixture(oracle, tmp_path):
     g = np.full(T * K, 0.5)
     for e in [0.1, 1.0, 10.0, 100.0]:
         g, _ = oracle.spg_solve(g, b, T, 1, K, e / T, maxit=100000, stop_difff=0, stop_kkt=1,
-                                kkt_tol=1e-12, kkt_scale=T, gd_centered=1, nthreads=8)
+                                kkt_tol=1e-12, kkt_scale=T, gd_centered=1, gd_certified=1, dad_edges=1, g_recompute=1, nthreads=8)
         errs[e] = (oracle.abserr_reconstructed(g, th, truth, T, 1, K), g.copy())
     e_or = min(errs, key=lambda k: errs[k][0])
     assert best_eps == e_or
@@ -90,7 +90,7 @@ def test_cpp_driver_fem_reduced_scan(oracle, tmp_path):
             g, _, _ = oracle.ts_solve(xs, g, th, T, 1, K, 1, e,
                                       ts=dict(theta_fixed=1, fem_reduce=r, fem_type=ftype),
                                       maxit=200000, stop_difff=0, stop_kkt=1, kkt_tol=1e-12,
-                                      kkt_scale=T * r, nthreads=8, gd_centered=1)
+                                      kkt_scale=T * r, nthreads=8, gd_centered=1, gd_certified=1, dad_edges=1, g_recompute=1)
             a_ref = oracle.abserr_reconstructed(g, th, truth, T, 1, K)
             print(f"fem type {ref_type} eps {e}: abserr device {a_dev} oracle {a_ref}")
             assert abs(a_dev - a_ref) <= 1e-4 * a_ref + 1e-3

@@ -26,7 +26,7 @@ def test_explicit_b_matches_oracle_minimiser(ctx, oracle, prec_name, K, eps):
     b = _rand_b(K * 7 + int(eps), T, K, 1.0 / T).ravel()  # t-major, 1/T-scaled like the reference
     g0 = np.full(T * K, 1.0 / K)
     ref, rr = oracle.spg_solve(g0, b, T, 1, K, eps / T, maxit=400000, stop_difff=0, stop_kkt=1,
-                               kkt_tol=1e-12, kkt_scale=T, nthreads=8, gd_centered=1)
+                               kkt_tol=1e-12, kkt_scale=T, nthreads=8, gd_centered=1, gd_certified=1, dad_edges=1, g_recompute=1)
     assert rr.kkt <= 1e-12
     x = np.zeros(T)  # the data are ignored once b is given
     P = Problem(ctx, x, K, precision=prec, epssqr=eps)
@@ -89,7 +89,7 @@ def test_explicit_b_on_grid(ctx, oracle):
     graph = grid_csr(w, h)
     ref, rr = oracle.spg_solve(np.full(R * K, 1.0 / K), b, 1, R, K, eps, graph=graph,
                                maxit=400000, stop_difff=0, stop_kkt=1, kkt_tol=1e-12,
-                               kkt_scale=1.0, nthreads=8, gd_centered=1)
+                               kkt_scale=1.0, nthreads=8, gd_centered=1, gd_certified=1, dad_edges=1, g_recompute=1)
     for prec in [capi.FP64, capi.FP32_BULK]:
         P = Problem(ctx, np.zeros(R), K, T=1, R=R, precision=prec, grid_width=w, epssqr=eps)
         P.set_linear_term(b)
@@ -112,7 +112,7 @@ def test_problem_without_data_is_defined_by_b(ctx, oracle):
     b = _rand_b(5, T, K, 1.0 / T).ravel()
     ref, rr = oracle.spg_solve(np.full(T * K, 1.0 / K), b, T, 1, K, coeff, maxit=400000,
                                stop_difff=0, stop_kkt=1, kkt_tol=1e-12, kkt_scale=T, nthreads=8,
-                               gd_centered=1)
+                               gd_centered=1, gd_certified=1, dad_edges=1, g_recompute=1)
     P = Problem(ctx, None, K, T=T, precision=capi.FP64, epssqr=coeff * T)
     P.set_gamma_uniform()
     with pytest.raises(capi.PascError):

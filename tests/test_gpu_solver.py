@@ -18,12 +18,14 @@ def _signal(seed, T, levels, p, sigma):
 
 
 def _oracle_min(oracle, x, theta, T, K, eps, tol=1e-12, graph=None, R=1, xdim=1, heldout=None):
-    """FP64 minimiser of the gamma QP (oracle SPG with the centred <g,d>, driven to KKT tol)."""
+    """FP64 minimiser of the gamma QP: the oracle SPG driven to KKT tol, with the sign-safe <g,d>,
+    d'Ad as a sum over edges and g recomputed from gamma (the op-by-op formulas stall near
+    KKT 1e-9 on some trajectories)."""
     b = oracle.assemble_b(x, theta, T, R, K, xdim=xdim, heldout=heldout)
     alpha = eps / T
     g0 = np.full(T * R * K, 1.0 / K)
     xs, r = oracle.spg_solve(g0, b, T, R, K, alpha, graph=graph, maxit=400000, stop_difff=0,
-                             stop_kkt=1, kkt_tol=tol, kkt_scale=T, nthreads=8, gd_centered=1)
+                             stop_kkt=1, kkt_tol=tol, kkt_scale=T, nthreads=8, gd_centered=1, gd_certified=1, dad_edges=1, g_recompute=1)
     assert r.kkt <= tol, "oracle minimiser did not converge"
     return xs, r, b
 
@@ -85,6 +87,33 @@ def test_fp64_iteration_semantics_follow_reference(ctx, oracle):
     assert abs(r.fx - rr.fx) <= 1e-8 * abs(rr.fx)
 
 
+@pytest.mark.parametrize("K", [2, 3, 4])
+def test_infeasible_start_and_stored_rows(ctx, oracle, K):
+    """The pass stores K - 1 rows (the last is rebuilt from the simplex constraint): from an
+    infeasible random gamma_0 (whose last row the first pass must read) the iterates follow
+    the oracle's, and the returned gamma has all K rows, non-negative and summing to 1."""
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Problem
+    T, eps = 3001, 10.0
+    levels = np.linspace(0.0, 3.0, K)
+    x, _ = _signal(30 + K, T, levels, 0.002, 0.5)
+    b = oracle.assemble_b(x, levels, T, 1, K)
+    g0 = np.random.default_rng(K).random(T * K) * 2.0 - 0.5  # t-major, off the simplex
+    for maxit in [1, 2, 7]:
+        xs, rr = oracle.spg_solve(g0, b, T, 1, K, eps / T, maxit=maxit, stop_difff=0)
+        P = Problem(ctx, x, K, precision=capi.FP64, epssqr=eps)
+        P.set_theta(levels)
+        P.set_gamma(g0)
+        r = P.gamma_solve(maxit=maxit, stop_mask=0)
+        g = P.get_gamma()
+        P.close()
+        assert r.it == maxit
+        assert np.max(np.abs(g - xs)) <= 1e-10
+        G = g.reshape(T, K)
+        assert G.min() >= 0.0 and np.max(np.abs(G.sum(axis=1) - 1.0)) <= 1e-14
+        assert abs(r.fx - rr.fx) <= 1e-12 * abs(rr.fx) + 1e-15
+
+
 @pytest.mark.parametrize("eps", [1.0, 100.0])
 def test_fp32_bulk_reaches_fp64_certified_kkt(ctx, oracle, eps):
     from pasc_inference_amd import capi
@@ -137,7 +166,7 @@ def test_alternation_c1_matches_oracle(ctx, oracle):
     x, truth = _signal(C1["seed"], C1["T"], C1["levels"], C1["p_switch"], C1["sigma"])
     T, K = C1["T"], 2
     spg = dict(maxit=100000, stop_difff=0, stop_kkt=1, kkt_tol=1e-11, kkt_scale=T, nthreads=8,
-               gd_centered=1)
+               gd_centered=1, gd_certified=1, dad_edges=1, g_recompute=1)
     g_o, th_o, tr_o = oracle.ts_solve(x, np.full(T * K, 0.5), C1["theta0"], T, 1, K, 1,
                                       C1["eps"], ts=dict(maxit=50, eps=1e-10,
                                                          skip_first_theta=1), **spg)
@@ -170,7 +199,7 @@ def test_grid_image_matches_oracle(ctx, oracle):
     b = oracle.assemble_b(x, theta, 1, R, K, xdim=3)
     xs, rr = oracle.spg_solve(np.full(R * K, 1.0 / K), b, 1, R, K, eps, graph=graph,
                               maxit=200000, stop_difff=0, stop_kkt=1, kkt_tol=1e-12,
-                              kkt_scale=1.0, nthreads=8, gd_centered=1)
+                              kkt_scale=1.0, nthreads=8, gd_centered=1, gd_certified=1, dad_edges=1, g_recompute=1)
     for prec in [capi.FP64, capi.FP32_BULK]:
         P = Problem(ctx, x, K, T=1, R=R, xdim=3, precision=prec, grid_width=w, epssqr=eps)
         P.set_theta(theta)

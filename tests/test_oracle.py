@@ -234,12 +234,39 @@ def test_spg_reaches_exact_minimisers_of_tiny_qps(oracle):
         b = np.array(c["b"])
         x, r = oracle.spg_solve(np.full(T * R * K, 1.0 / K), b, T, R, K, a, graph=graph,
                                 maxit=20000, stop_difff=0, stop_kkt=1, kkt_tol=1e-13,
-                                kkt_scale=1.0, gd_centered=1, safeguard=1)
+                                kkt_scale=1.0, gd_centered=1, gd_certified=1, dad_edges=1, g_recompute=1, safeguard=1)
         assert abs(r.fx - c["f"]) <= 1e-10 * max(1.0, abs(c["f"])), c
         assert abs(oracle.objective(x, b, T, R, K, a, graph=graph) - c["f"]) <= 1e-10
         # the minimiser is unique when A is definite on the feasible face; compare where the
         # brute force and SPG agree on the support
         np.testing.assert_allclose(x, c["gamma"], atol=1e-6)
+
+
+@pytest.mark.parametrize("R,graph", [(1, None), (6, "grid")])
+def test_robust_dot_forms_equal_op_by_op(oracle, R, graph):
+    """The minimiser options (certified <g,d>, d'Ad over edges, g recomputed) change only the
+    rounding: one iteration from the same start gives the same dAd and f as the op-by-op
+    formulas, and both runs reach the same minimiser."""
+    T, K, eps = 400, 3, 5.0
+    rng = np.random.default_rng(3)
+    csr = grid_csr(3, 2) if graph else None
+    b = -rng.random(T * R * K) / T
+    g0 = rng.random(T * R * K)
+    base = dict(stop_difff=0, nthreads=2)
+    rob = dict(base, gd_certified=1, dad_edges=1, g_recompute=1)
+    for maxit in (1, 2):
+        _, r0 = oracle.spg_solve(g0, b, T, R, K, eps / T, graph=csr, maxit=maxit, **base)
+        _, r1 = oracle.spg_solve(g0, b, T, R, K, eps / T, graph=csr, maxit=maxit, **rob)
+        assert r1.dAd == pytest.approx(r0.dAd, rel=1e-11)
+        assert r1.dd == pytest.approx(r0.dd, rel=1e-12)
+        assert r1.fx == pytest.approx(r0.fx, rel=1e-13)
+    x0, r0 = oracle.spg_solve(g0, b, T, R, K, eps / T, graph=csr, maxit=200000, stop_kkt=1,
+                              kkt_tol=1e-10, kkt_scale=T, **dict(base, gd_centered=1))
+    x1, r1 = oracle.spg_solve(g0, b, T, R, K, eps / T, graph=csr, maxit=200000, stop_kkt=1,
+                              kkt_tol=1e-12, kkt_scale=T, **rob)
+    assert r0.kkt <= 1e-10 and r1.kkt <= 1e-12
+    assert np.max(np.abs(x0 - x1)) <= 1e-7
+    assert r1.fx == pytest.approx(r0.fx, rel=1e-12)
 
 
 def test_reference_divides_by_zero_when_A_vanishes(oracle):
@@ -351,7 +378,7 @@ def test_lloyd_limit_epsilon_zero(oracle):
     x = th[rng.integers(0, K, T)] + 0.3 * rng.normal(size=T)
     b = oracle.assemble_b(x, th, T, 1, K)
     g, r = oracle.spg_solve(np.full(T * K, 1 / 3), b, T, 1, K, 0.0, maxit=200000, stop_difff=0,
-                            stop_kkt=1, kkt_tol=1e-12, kkt_scale=T, gd_centered=1, safeguard=1)
+                            stop_kkt=1, kkt_tol=1e-12, kkt_scale=T, gd_centered=1, gd_certified=1, dad_edges=1, g_recompute=1, safeguard=1)
     assert r.stop_reason == 6
     lab = np.argmin((x[:, None] - th[None]) ** 2, 1)
     np.testing.assert_allclose(g.reshape(T, K), np.eye(K)[lab], atol=1e-9)
@@ -365,7 +392,7 @@ def test_large_epsilon_limit_constant_gamma(oracle):
     x = rng.normal(size=T) * 0.5 + 0.3
     b = oracle.assemble_b(x, th, T, 1, K)
     g, r = oracle.spg_solve(np.full(T * K, 0.5), b, T, 1, K, 1e6 / T, maxit=200000,
-                            stop_difff=0, stop_kkt=1, kkt_tol=1e-10, kkt_scale=T, gd_centered=1)
+                            stop_difff=0, stop_kkt=1, kkt_tol=1e-10, kkt_scale=T, gd_centered=1, gd_certified=1, dad_edges=1, g_recompute=1)
     G = g.reshape(T, K)
     assert np.ptp(G, axis=0).max() < 1e-6
     # the constant is the LP optimum: all weight on the cluster with the smaller data cost
@@ -384,7 +411,7 @@ def test_ts_alternation_monotone_on_reference_fixture(oracle):
     g, th, r = oracle.ts_solve(x, np.full(T * K, 0.5), [0.8, 2.2], T, 1, K, 1, 10.0,
                                ts=dict(maxit=50, eps=1e-9, skip_first_theta=1),
                                maxit=20000, stop_difff=0, stop_kkt=1, kkt_tol=1e-9,
-                               kkt_scale=T, gd_centered=1, nthreads=4)
+                               kkt_scale=T, gd_centered=1, gd_certified=1, dad_edges=1, g_recompute=1, nthreads=4)
     assert r.increased == 0
     np.testing.assert_allclose(th, [1.0, 2.0], atol=0.05)  # clusters keep the order of theta0
     lab = g.reshape(T, K).argmax(1)
@@ -410,7 +437,7 @@ def test_labels_match_reference_output_fixture(oracle):
     for eps in [0.1, 1.0, 10.0, 100.0]:
         g, r = oracle.spg_solve(np.full(T * K, 0.5), b, T, 1, K, eps / T, maxit=50000,
                                 stop_difff=0, stop_kkt=1, kkt_tol=1e-8, kkt_scale=T,
-                                gd_centered=1, nthreads=4)
+                                gd_centered=1, gd_certified=1, dad_edges=1, g_recompute=1, nthreads=4)
         ae = oracle.abserr_reconstructed(g, th, truth, T, 1, K)
         if best is None or ae < best[0]:
             best = (ae, g)
@@ -445,7 +472,7 @@ def test_unrolled_spg_reference_matches_central_differences(oracle):
         b = oracle.assemble_b(x, th, T, 1, K, heldout=mask)
         g, _ = oracle.spg_solve(np.full(T * K, 1.0 / K), b, T, 1, K, np.exp(le) / T,
                                 maxit=100000, stop_difff=0, stop_kkt=1, kkt_tol=1e-13,
-                                kkt_scale=T, gd_centered=1)
+                                kkt_scale=T, gd_centered=1, gd_certified=1, dad_edges=1, g_recompute=1)
         return float(np.sum(((x - g.reshape(T, K) @ th)[mask == 1]) ** 2))
 
     le = np.log(eps)

@@ -104,7 +104,7 @@ def test_fem_alternation_bookkeeping(oracle):
     x, truth = generate_markov(0, T, [1.0, 2.0], 0.01, 0.4)
     theta0 = np.array([0.8, 2.2])
     spg = dict(maxit=20000, stop_difff=0, stop_kkt=1, kkt_tol=1e-10, kkt_scale=1.0, nthreads=4,
-               gd_centered=1)
+               gd_centered=1, gd_certified=1, dad_edges=1, g_recompute=1)
     for ftype in (oracle.FEM_CONSTANT, oracle.FEM_HAT):
         g0 = np.full(T * K, 1.0 / K)
         ts = dict(maxit=4, eps=0.0, skip_first_theta=1, fem_reduce=r, fem_type=ftype)

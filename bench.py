@@ -280,7 +280,9 @@ def run_cpu_baseline(args, c2):
 
 
 def profile_run(args):
-    """Fixed work for profilers: 64 fused FP32 passes, 64 FP64 passes, a 1 GiB copy kernel.
+    """Fixed work for profilers: a 1 GiB copy kernel, then one solve of the bench workload per
+    path (FP32-bulk with its FP64 polish, then FP64) to the bench's KKT target, so that per-pass
+    averages cover a whole trajectory as the bench's own pass timing does.
 
     With --tune-cache the passes use the launch shapes an earlier bench run recorded there."""
     from pasc_inference_amd import capi
@@ -296,11 +298,13 @@ def profile_run(args):
             if args.tune_cache:
                 P.autotune(args.tune_cache)
             P.set_profiling(True)
-            r = P.gamma_solve(maxit=64, stop_mask=0, polish_maxit=1, check_every=32, kkt_every=4)
+            r = P.gamma_solve(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt,
+                              check_every=32, kkt_every=4)
             P.close()
             t = r.pass_time_fp32_s if prec == capi.FP32_BULK else r.pass_time_fp64_s
             n = r.passes_fp32 if prec == capi.FP32_BULK else r.passes_fp64
             out[name + "_pass_us"] = t / max(n, 1) * 1e6
+            out[name + "_passes"] = n
     print(json.dumps(out))
 
 

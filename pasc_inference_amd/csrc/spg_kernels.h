@@ -102,14 +102,15 @@ __device__ __forceinline__ void exchange_edges(const C (&v)[K][CH], C *lds, C (&
 }
 
 template <typename C>
-__device__ __forceinline__ C lap2(C v, C vm, C vp, int64_t t, int64_t T, bool interior) {
-    // row of A0 (blockgraphsparse.cpp:60-89) in difference form:
-    // 2*(v_t - v_{t-1})[t>0] + 2*(v_t - v_{t+1})[t<T-1]
-    if (interior) return C(2) * ((v - vm) + (v - vp));
+__device__ __forceinline__ C lap_half(C v, C vm, C vp, int64_t t, int64_t T, bool interior) {
+    // half of row t of A0 (blockgraphsparse.cpp:60-89) in difference form:
+    // (v_t - v_{t-1})[t>0] + (v_t - v_{t+1})[t<T-1]; the callers scale by 2*alpha (the factor
+    // 2 is exact, so (2 alpha) * s equals alpha * (2 s) bit for bit, one multiply fewer)
+    if (interior) return (v - vm) + (v - vp);
     C a = C(0);
     if (t > 0) a += v - vm;
     if (t < T - 1) a += v - vp;
-    return C(2) * a;
+    return a;
 }
 
 // One tile of the fused pass.  INT: every step of the tile, halos included, lies in [1, T-2]
@@ -241,6 +242,7 @@ __device__ __forceinline__ void pass_tile_1d(const TileCtx<S, C, K> &cx, int chu
     const S *src = cx.src;
     S *dst = cx.dst;
     const C alpha = cx.alpha;
+    const C alpha2 = C(2) * alpha;
     const C bscale = cx.bscale;
     const C s_kkt = cx.s_kkt;
     const C a_apply = cx.a_apply;
@@ -336,7 +338,7 @@ __device__ __forceinline__ void pass_tile_1d(const TileCtx<S, C, K> &cx, int chu
                 for (int k = 0; k < K; k++) {
                     const C vm = c > 0 ? G[k][c - 1] : gl[k];
                     const C vp = c < CH - 1 ? G[k][c + 1] : gr[k];
-                    const C g = alpha * lap2(G[k][c], vm, vp, t, T, interior) + Q[k][c];
+                    const C g = alpha2 * lap_half(G[k][c], vm, vp, t, T, interior) + Q[k][c];
                     y[k] = G[k][c] - a_apply * g;
                 }
                 project_simplex<K, C>(y);
@@ -360,7 +362,7 @@ __device__ __forceinline__ void pass_tile_1d(const TileCtx<S, C, K> &cx, int chu
             for (int k = 0; k < K; k++) {
                 const C vm = c > 0 ? G1[k][c - 1] : gl[k];
                 const C vp = c < CH - 1 ? G1[k][c + 1] : gr[k];
-                g[k] = alpha * lap2(G1[k][c], vm, vp, t, T, interior) + Q[k][c];
+                g[k] = alpha2 * lap_half(G1[k][c], vm, vp, t, T, interior) + Q[k][c];
                 y[k] = G1[k][c] - a_dir * g[k];
                 z[k] = y[k];
             }
@@ -373,7 +375,8 @@ __device__ __forceinline__ void pass_tile_1d(const TileCtx<S, C, K> &cx, int chu
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 Dd[k][c] = z[k] - G1[k][c];
-                if (y[k] <= tau) o += (tau - y[k]) * G1[k][c];
+                // (tau - y_k)^+ gamma_k: the off-support terms (y_k <= tau), as one max
+                o += cmax(tau - y[k], C(0)) * G1[k][c];
             }
             if (owner && (INT || t < T)) {
                 p_off += o;

@@ -28,7 +28,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 def _units():
     units = []
     for s in ("float", "double"):
-        for nt in (128, 256, 512):
+        for nt in (64, 128, 256, 512):
             units.append((os.path.join(CSRC, "pass_inst.hip"), f"pass_{s}_{nt}.o",
                           [f"-DPASS_S={s}", f"-DPASS_NT={nt}"]))
         units.append((os.path.join(CSRC, "grid_inst.hip"), f"grid_{s}.o", [f"-DPASS_S={s}"]))

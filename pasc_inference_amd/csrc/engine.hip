@@ -33,9 +33,11 @@ static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 // kernel dispatch tables
 // ------------------------------------------------------------------------------------------
 // instantiated in pass_inst.hip / grid_inst.hip (one object per storage type and block size)
+int launch_pass_1d_float_64_(int K, const PassArgs &a, int grid, int chunk, int pf, hipStream_t s);
 int launch_pass_1d_float_128_(int K, const PassArgs &a, int grid, int chunk, int pf, hipStream_t s);
 int launch_pass_1d_float_256_(int K, const PassArgs &a, int grid, int chunk, int pf, hipStream_t s);
 int launch_pass_1d_float_512_(int K, const PassArgs &a, int grid, int chunk, int pf, hipStream_t s);
+int launch_pass_1d_double_64_(int K, const PassArgs &a, int grid, int chunk, int pf, hipStream_t s);
 int launch_pass_1d_double_128_(int K, const PassArgs &a, int grid, int chunk, int pf, hipStream_t s);
 int launch_pass_1d_double_256_(int K, const PassArgs &a, int grid, int chunk, int pf, hipStream_t s);
 int launch_pass_1d_double_512_(int K, const PassArgs &a, int grid, int chunk, int pf, hipStream_t s);
@@ -45,10 +47,12 @@ int launch_pass_grid_double(int K, const GridArgs &a, int grid, hipStream_t s);
 static int launch_pass_1d(bool f64, int K, const PassArgs &a, int grid, int block, int chunk,
                           int pf, hipStream_t s) {
     if (f64) {
+        if (block == 64) return launch_pass_1d_double_64_(K, a, grid, chunk, pf, s);
         if (block == 128) return launch_pass_1d_double_128_(K, a, grid, chunk, pf, s);
         if (block == 256) return launch_pass_1d_double_256_(K, a, grid, chunk, pf, s);
         if (block == 512) return launch_pass_1d_double_512_(K, a, grid, chunk, pf, s);
     } else {
+        if (block == 64) return launch_pass_1d_float_64_(K, a, grid, chunk, pf, s);
         if (block == 128) return launch_pass_1d_float_128_(K, a, grid, chunk, pf, s);
         if (block == 256) return launch_pass_1d_float_256_(K, a, grid, chunk, pf, s);
         if (block == 512) return launch_pass_1d_float_512_(K, a, grid, chunk, pf, s);
@@ -986,7 +990,7 @@ int engine_fill_uniform(Engine &e) {
 
 static int check_shape(const PassShape &s, bool f32) {
     const bool ok_chunk = f32 ? (s.chunk == 4 || s.chunk == 8) : (s.chunk == 2 || s.chunk == 4);
-    if (!(s.block == 128 || s.block == 256 || s.block == 512) || !ok_chunk ||
+    if (!(s.block == 64 || s.block == 128 || s.block == 256 || s.block == 512) || !ok_chunk ||
         (s.prefetch != 0 && s.prefetch != 1))
         return fail(PASC_ERR_ARG, "no fused %s pass for block %d chunk %d prefetch %d",
                     f32 ? "FP32" : "FP64", s.block, s.chunk, s.prefetch);
@@ -1027,7 +1031,7 @@ int engine_autotune(Engine &e, PassShape *s32, PassShape *s64, double us[2]) {
     }
     const bool f32 = e.precision == PASC_FP32_BULK;
     const int chunks32[2] = {4, 8}, chunks64[2] = {2, 4};
-    const int blocks[3] = {128, 256, 512};
+    const int blocks[4] = {64, 128, 256, 512};
     const int slot_opts[4][2] = {{2048, 1}, {1024, 1}, {512, 1}, {2048, 4}};
     pasc_spg_params p;
     pasc_spg_default_params(&p);
@@ -1042,7 +1046,7 @@ int engine_autotune(Engine &e, PassShape *s32, PassShape *s64, double us[2]) {
     for (int pi = 0; pi < 2; pi++)
     for (int si = 0; si < (pi ? 4 : 1); si++)
     for (int ci = 0; ci < 2; ci++)
-    for (int bi = 0; bi < 3; bi++) {
+    for (int bi = 0; bi < 4; bi++) {
         const PassShape c32{blocks[bi], chunks32[ci], pi, slot_opts[si][0], slot_opts[si][1]};
         const PassShape c64{blocks[bi], chunks64[ci], pi, slot_opts[si][0], slot_opts[si][1]};
         e.sh[0] = c32;

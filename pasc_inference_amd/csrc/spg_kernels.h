@@ -82,11 +82,23 @@ __device__ __forceinline__ void load_chunk(const S *__restrict__ row, int64_t t0
 }
 
 // Neighbour values across chunk edges: every thread publishes its first and last step of each
-// row; thread i then reads step t0-1 from thread i-1 and step t0+CH from thread i+1.
+// row; thread i then reads step t0-1 from thread i-1 and step t0+CH from thread i+1.  A
+// one-wave workgroup (NT = 64) moves them with cross-lane shuffles: no LDS round trip and no
+// barrier, so its waves never wait for each other.
 template <typename C, int K, int CH, int NT>
 __device__ __forceinline__ void exchange_edges(const C (&v)[K][CH], C *lds, C (&left)[K],
                                                C (&right)[K]) {
     const int tid = threadIdx.x;
+    if constexpr (NT == WAVE) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const C l = __shfl_up(v[k][CH - 1], 1, WAVE);
+            const C r = __shfl_down(v[k][0], 1, WAVE);
+            left[k] = tid > 0 ? l : C(0);
+            right[k] = tid < NT - 1 ? r : C(0);
+        }
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < K; k++) {
         lds[(2 * k) * NT + tid] = v[k][0];
@@ -322,7 +334,10 @@ __device__ __forceinline__ void pass_tile_1d(const TileCtx<S, C, K> &cx, int chu
         }
         C gl[K], gr[K];
         exchange_edges<C, K, CH, NT>(G, lds_x, gl, gr);
-        next();  // the exchange's barriers order every stage read before the next copies
+        // every stage read must be done before the next copies overwrite the stage: the
+        // exchange's barriers see to that, and a one-wave workgroup waits for its own reads
+        if constexpr (NT == WAVE && STAGED) asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        next();
 #pragma unroll
         for (int c = 0; c < CH; c++) {
             const int64_t t = t0 + c;

@@ -115,11 +115,15 @@ __device__ __forceinline__ void step_instance(InstState *state, int ntiles, doub
                                               const StepParams &P, int32_t *dirty,
                                               InstState *lds_state, double *lds_red,
                                               LoadSlot load_slot) {
-    static_assert(STATE_WORDS <= NT, "one state word per thread");
+    constexpr int WPT = (STATE_WORDS + NT - 1) / NT;  // state words per thread
     uint64_t *gw = reinterpret_cast<uint64_t *>(state);
     uint64_t *lw = reinterpret_cast<uint64_t *>(lds_state);
-    uint64_t word = 0;
-    if (threadIdx.x < STATE_WORDS) word = gw[threadIdx.x];
+    uint64_t word[WPT];
+#pragma unroll
+    for (int q = 0; q < WPT; q++) {
+        const int w = threadIdx.x + q * NT;
+        word[q] = w < STATE_WORDS ? gw[w] : 0;
+    }
     double v[NPART] = {0.0, 0.0, 0.0, 0.0, -1.0};  // residual -1: not evaluated
     // B slots per thread are loaded before any is used, so that their latencies overlap
     constexpr int B = 4;
@@ -147,11 +151,15 @@ __device__ __forceinline__ void step_instance(InstState *state, int ntiles, doub
             v[NPART - 1] = (m > v[NPART - 1] || m != m) ? m : v[NPART - 1];
         }
     }
-    if (threadIdx.x < STATE_WORDS) lw[threadIdx.x] = word;
+#pragma unroll
+    for (int q = 0; q < WPT; q++) {
+        const int w = threadIdx.x + q * NT;
+        if (w < STATE_WORDS) lw[w] = word[q];
+    }
     block_reduce_part<NT>(v, lds_red);  // its barriers also publish the staged state
     if (threadIdx.x == 0) step_scalar(sample_eps, *lds_state, v, P, dirty);
     __syncthreads();
-    if (threadIdx.x < STATE_WORDS) gw[threadIdx.x] = lw[threadIdx.x];
+    for (int w = threadIdx.x; w < STATE_WORDS; w += NT) gw[w] = lw[w];
 }
 
 

@@ -263,7 +263,7 @@ def test_autotune_keeps_iterate_and_results(ctx, oracle):
         sh = P.autotune()
         print(f"prec={prec}: {sh}")
         m = sh["main"]
-        assert m["block"] in (128, 256, 512) and m["prefetch"] in (0, 1) and m["us_per_pass"] > 0
+        assert m["block"] in (64, 128, 256, 512) and m["prefetch"] in (0, 1) and m["us_per_pass"] > 0
         if prec == capi.FP32_BULK:
             assert m["chunk"] in (4, 8) and sh["polish"]["chunk"] in (2, 4)
             assert sh["polish"]["us_per_pass"] > 0
@@ -293,7 +293,7 @@ def test_launch_shapes_give_identical_iterates(ctx, prec_name):
     chunks = (4, 8) if prec == capi.FP32_BULK else (2, 4)
     P = Problem(ctx, x, K, precision=prec, epssqr=eps)
     P.set_theta(theta)
-    for block in (128, 256):
+    for block in (64, 128, 256):
         for chunk in chunks:
             tilings = [(3, 0), (0, 0), (0, 4)]  # (max_slots, min_tiles)
             outs = {}
@@ -310,6 +310,41 @@ def test_launch_shapes_give_identical_iterates(ctx, prec_name):
                 assert a[0] == b[0] and np.array_equal(a[2], b[2]), (slots, mt)
     with pytest.raises(capi.PascError):
         P.set_launch_shape(192, chunks[0])
+
+
+@pytest.mark.parametrize("prec_name,chunk", [("FP64", 2), ("FP64", 4), ("FP32_BULK", 4), ("FP32_BULK", 8)])
+def test_one_wave_workgroups_follow_oracle(ctx, oracle, prec_name, chunk):
+    """64-thread workgroups exchange chunk edges by cross-lane shuffles instead of LDS and
+    barriers: the iterates follow the oracle's (FP64: to rounding; FP32: to FP32 rounding) and
+    the solve reaches the oracle's minimiser."""
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Problem
+    prec = getattr(capi, prec_name)
+    T, K, eps = 4001, 3, 10.0
+    levels = [0.0, 1.0, 3.0]
+    x, _ = _signal(41, T, levels, 2e-3, 0.6)
+    theta = np.array(levels)
+    b = oracle.assemble_b(x, theta, T, 1, K)
+    P = Problem(ctx, x, K, precision=prec, epssqr=eps)
+    P.set_theta(theta)
+    P.set_launch_shape(64, chunk, 1, 0, 0, polish=dict(block=64, chunk=2, prefetch=1))
+    if prec == capi.FP64:
+        for maxit in (1, 3, 20):
+            xs, rr = oracle.spg_solve(np.full(T * K, 1.0 / K), b, T, 1, K, eps / T, maxit=maxit,
+                                      stop_difff=0)
+            P.set_gamma_uniform()
+            r = P.gamma_solve(maxit=maxit, stop_mask=0)
+            assert r.it == maxit
+            assert np.max(np.abs(P.get_gamma() - xs)) <= 1e-10
+            assert abs(r.fx - rr.fx) <= 1e-12 * abs(rr.fx)
+    ref, _, _ = _oracle_min(oracle, x, theta, T, K, eps)
+    P.set_gamma_uniform()
+    r = P.gamma_solve(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=1e-11)
+    g = P.get_gamma()
+    print(f"{prec_name} chunk {chunk}: it={r.it} kkt={r.kkt:.2e} err={np.max(np.abs(g - ref)):.2e}")
+    assert r.stop_reason == capi.REASON_KKT and r.kkt <= 1e-11
+    assert np.max(np.abs(g - ref)) <= 1e-6
+    P.close()
 
 
 def test_autotune_file_cache(tmp_path):

@@ -296,7 +296,7 @@ int pasc_device_copy_bandwidth(pasc_ctx *ctx, int64_t bytes, int32_t reps, doubl
 
 /* ---- launch shape of the fused 1-D pass ---- */
 typedef struct {
-    int32_t block;       /* threads per workgroup: 128, 256 or 512 */
+    int32_t block;       /* threads per workgroup: 64 (one wave), 128, 256 or 512 */
     int32_t chunk;       /* time steps per thread: FP32 passes 4 or 8, FP64 passes 2 or 4 */
     int32_t prefetch;    /* 1: asynchronous LDS staging of a workgroup's next tile */
     int32_t max_slots;   /* partial-sum slots per instance at most (0 = 2048); each slot is one

@@ -81,10 +81,26 @@ __device__ __forceinline__ void load_chunk(const S *__restrict__ row, int64_t t0
     }
 }
 
+// Whole-wave lane shifts by one lane (DPP wave_shr:1 / wave_shl:1, VALU moves): lane i gets
+// lane i-1 (from_left) or lane i+1 (from_right); the lane without a source gets 0.
+template <int CTRL>
+__device__ __forceinline__ float dpp_shift(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift(double v) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(u & 0xffffffffll), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)(unsigned)hi << 32) | (long long)(unsigned)lo);
+}
+constexpr int DPP_WAVE_SHR1 = 0x138;  // lane i <- lane i-1
+constexpr int DPP_WAVE_SHL1 = 0x130;  // lane i <- lane i+1
+
 // Neighbour values across chunk edges: every thread publishes its first and last step of each
 // row; thread i then reads step t0-1 from thread i-1 and step t0+CH from thread i+1.  A
-// one-wave workgroup (NT = 64) moves them with cross-lane shuffles: no LDS round trip and no
-// barrier, so its waves never wait for each other.
+// one-wave workgroup (NT = 64) moves them with DPP lane shifts: no LDS and no barrier, so its
+// waves never wait for each other.
 template <typename C, int K, int CH, int NT>
 __device__ __forceinline__ void exchange_edges(const C (&v)[K][CH], C *lds, C (&left)[K],
                                                C (&right)[K]) {
@@ -92,10 +108,8 @@ __device__ __forceinline__ void exchange_edges(const C (&v)[K][CH], C *lds, C (&
     if constexpr (NT == WAVE) {
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            const C l = __shfl_up(v[k][CH - 1], 1, WAVE);
-            const C r = __shfl_down(v[k][0], 1, WAVE);
-            left[k] = tid > 0 ? l : C(0);
-            right[k] = tid < NT - 1 ? r : C(0);
+            left[k] = dpp_shift<DPP_WAVE_SHR1>(v[k][CH - 1]);
+            right[k] = dpp_shift<DPP_WAVE_SHL1>(v[k][0]);
         }
         return;
     }

@@ -483,13 +483,32 @@ __device__ __forceinline__ void pass_tile_1d(const TileCtx<S, C, K> &cx, int chu
 // The step of one instance, run by the workgroup that completed its last partial slot: the
 // slots are reduced in a fixed order and step_scalar advances the instance.
 template <int NT>
-__device__ __forceinline__ void fused_step(const PassArgs &A, int inst, InstState *lds_state,
-                                           double *lds_red) {
-    const InstDesc &D = A.desc[inst];
-    const double *part = A.part + (int64_t)uni(D.tile0) * NPART;
-    step_instance<NT>(A.state + inst, uni(D.ntiles), D.alpha * D.s_kkt, A.sp, A.dirty, lds_state,
+__device__ __forceinline__ void fused_step(const InstDesc *desc, InstState *state,
+                                           const double *parts, const StepParams &sp,
+                                           int32_t *dirty, int32_t *nactive, int inst,
+                                           InstState *lds_state, double *lds_red) {
+    const InstDesc &D = desc[inst];
+    const double *part = parts + (int64_t)uni(D.tile0) * NPART;
+    step_instance<NT>(state + inst, uni(D.ntiles), D.alpha * D.s_kkt, sp, dirty, lds_state,
                       lds_red, [&](int i, int q) { return load_coherent(part + (int64_t)i * NPART + q); });
-    if (threadIdx.x == 0 && !lds_state->active) atomicSub(A.nactive, 1);
+    if (threadIdx.x == 0 && !lds_state->active) atomicSub(nactive, 1);
+}
+
+// Counts one completed partial slot of a single problem (its values already written with
+// store_coherent and publish_wait): slot j on sub-counter j % DONE_SUB, the last arrival of a
+// sub-counter on the top counter.  True in the workgroup that completed the instance's last
+// slot; every counter is back to 0 afterwards.  Thread 0 only.
+__device__ __forceinline__ bool count_slot(int32_t *done, int slot, int ns) {
+    const int g = slot % DONE_SUB;
+    const int n_g = ns / DONE_SUB + (g < ns % DONE_SUB ? 1 : 0);
+    int32_t *sub = done + g * DONE_STRIDE;
+    if (atomicAdd(sub, 1) != n_g - 1) return false;
+    __hip_atomic_store(sub, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int32_t *top = done + DONE_SUB * DONE_STRIDE;
+    const int n_top = ns < DONE_SUB ? ns : DONE_SUB;
+    if (atomicAdd(top, 1) != n_top - 1) return false;
+    __hip_atomic_store(top, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
 }
 
 // PF: software pipelining across the msub tiles of a slot.  While a tile is computed, the
@@ -569,19 +588,7 @@ __global__ __launch_bounds__(NT) void spg_pass_1d(PassArgs A) {
 #pragma unroll
                 for (int i = 0; i < NPART; i++) store_coherent(o + i, acc[i]);
                 publish_wait();
-                const int ns = uni(A.desc[inst].ntiles);
-                const int g = slot % DONE_SUB;
-                const int n_g = ns / DONE_SUB + (g < ns % DONE_SUB ? 1 : 0);
-                int32_t *sub = A.done + g * DONE_STRIDE;
-                if (atomicAdd(sub, 1) == n_g - 1) {
-                    __hip_atomic_store(sub, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    int32_t *top = A.done + DONE_SUB * DONE_STRIDE;
-                    const int n_top = ns < DONE_SUB ? ns : DONE_SUB;
-                    if (atomicAdd(top, 1) == n_top - 1) {
-                        __hip_atomic_store(top, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        finished = inst;
-                    }
-                }
+                if (count_slot(A.done, slot, uni(A.desc[inst].ntiles))) finished = inst;
             } else {
 #pragma unroll
                 for (int i = 0; i < NPART; i++) o[i] = acc[i];
@@ -592,7 +599,9 @@ __global__ __launch_bounds__(NT) void spg_pass_1d(PassArgs A) {
     if (A.done) {
         __syncthreads();
         const int fin = finished;
-        if (fin >= 0) fused_step<NT>(A, fin, &lds_state, lds_red);
+        if (fin >= 0)
+            fused_step<NT>(A.desc, A.state, A.part, A.sp, A.dirty, A.nactive, fin, &lds_state,
+                           lds_red);
     }
 }
 
@@ -670,6 +679,9 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_dir(GridArgs A) {
         const InstState &st = A.state[inst];
         const int64_t N = D.T;  // nodes T*R
         const int par = st.parity;
+        // the KKT residual on every pass: these kernels are bound by their neighbour loads, the
+        // extra projection costs nothing measurable, and the solve stops at the first pass that
+        // reaches the target (the residual is not monotone along SPG)
         double acc[NPART] = {0.0, 0.0, 0.0, 0.0, 0.0};
       for (int m = 0; m < D.msub; m++) {
         const int64_t r = (int64_t)(slot * D.msub + m) * NT + threadIdx.x;

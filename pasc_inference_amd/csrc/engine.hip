@@ -79,7 +79,7 @@ static int engine_retile(Engine &e, bool f64_phase) {
     int slots = 0;
     for (auto &d : e.h_desc) {
         // at most max_slots (default 2048) partial slots per instance: enough workgroups to fill
-        // the device, few enough that the step kernel reduces them quickly; at least min_tiles
+        // the device, few enough that the step reduces them quickly; at least min_tiles
         // tiles per slot
         const int64_t raw = (d.T + tl - 1) / tl;
         const int64_t ms = std::max(sh.max_slots, 1);

@@ -47,7 +47,8 @@ struct InstDesc {
     double theta[PASC_MAX_K * 3];  // theta[k*xdim + c], xdim <= 3
 };
 
-// SPG state of one instance, owned by the device step kernel.
+// SPG state of one instance, advanced on the device by step_scalar (in the step kernel, or in
+// the 1-D pass's last workgroup for single problems).
 struct InstState {
     double f, f_prev;
     double dd_prev, dAd_prev;  // dots of the direction of the last completed iteration
@@ -76,7 +77,7 @@ constexpr int DONE_SUB = 64;
 constexpr int DONE_STRIDE = 32;  // int32 words per counter
 constexpr int DONE_WORDS = (DONE_SUB + 1) * DONE_STRIDE;
 
-// Solver constants copied to the step kernel.
+// Solver constants of the step (kernel arguments of the step kernel and of the fused pass).
 struct StepParams {
     int32_t maxit;
     int32_t m;

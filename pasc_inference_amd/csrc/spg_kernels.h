@@ -13,7 +13,9 @@
 //
 // The time stencil has radius 1 and is applied three times (g_{i-1}, g_i, Ad_i), so each
 // thread owns CH consecutive time steps and recomputes a 3-step halo on each side in registers
-// (neighbouring threads load the same cache lines; HBM sees each element once).
+// (neighbouring threads load the same cache lines; HBM sees each element once).  Only K - 1
+// rows of gamma are read and written (the last is 1 - the others), and for a single problem
+// the workgroup that completes the pass also runs the step, so an iteration is one launch.
 #pragma once
 
 #include "device_common.h"

@@ -397,3 +397,34 @@ def test_batch_autotune_keeps_results(ctx):
             j = s * 2 + e
             assert r0[j].kkt <= 1e-9 and r1[j].kkt <= 1e-9
             assert np.max(np.abs(B0.get_gamma(s, e) - B1.get_gamma(s, e))) <= 1e-6
+
+
+@pytest.mark.parametrize("prec_name", ["FP64", "FP32_BULK"])
+def test_tiny_and_wide_problems(ctx, oracle, prec_name):
+    """Edge shapes of the 1-D pass: T from 1 to a few hundred steps (one partial slot, boundary
+    tiles only) and K from 1 to 8 (K = 1 stores its single row; K >= 2 stores K - 1 rows and
+    rebuilds the last): every solve reaches the oracle's minimiser."""
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Problem
+    prec = getattr(capi, prec_name)
+    rng = np.random.default_rng(12)
+    for T in (1, 2, 3, 5, 130):
+        for K in (1, 2, 5, 8):
+            levels = np.linspace(0.0, 2.0, K)
+            x = rng.normal(0.0, 0.7, T) + levels[rng.integers(0, K, T)]
+            eps = 3.0
+            b = oracle.assemble_b(x, levels, T, 1, K)
+            ref, rr = oracle.spg_solve(np.full(T * K, 1.0 / K), b, T, 1, K, eps / T,
+                                       maxit=200000, stop_difff=0, stop_kkt=1, kkt_tol=1e-12,
+                                       kkt_scale=T, gd_centered=1, gd_certified=1, dad_edges=1,
+                                       g_recompute=1, safeguard=1)
+            P = Problem(ctx, x, K, precision=prec, epssqr=eps)
+            P.set_theta(levels)
+            P.set_gamma_uniform()
+            r = P.gamma_solve(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=1e-10)
+            g = P.get_gamma()
+            P.close()
+            G = g.reshape(T, K)
+            assert r.status == 0 and r.kkt <= 1e-10, (T, K, r.kkt)
+            assert G.min() >= 0.0 and np.max(np.abs(G.sum(axis=1) - 1.0)) <= 1e-12, (T, K)
+            assert np.max(np.abs(g - ref)) <= 1e-6, (T, K, np.max(np.abs(g - ref)))

@@ -157,8 +157,8 @@ int engine_alloc(Engine &e, Ctx *ctx, int precision, int stencil, int32_t grid_w
     HIP_TRY(hipMalloc(&e.d_desc, sizeof(InstDesc) * std::max(e.n_inst, 1)));
     HIP_TRY(hipMalloc(&e.d_state, sizeof(InstState) * std::max(e.n_inst, 1)));
     HIP_TRY(hipMalloc(&e.d_nactive, 2 * sizeof(int32_t)));  // [0] active count, [1] dirty flag
-    HIP_TRY(hipMalloc(&e.d_done, sizeof(int32_t) * DONE_WORDS));
-    HIP_TRY(hipMemsetAsync(e.d_done, 0, sizeof(int32_t) * DONE_WORDS, S_(e)));
+    HIP_TRY(hipMalloc(&e.d_done, DONE_BYTES));
+    HIP_TRY(hipMemsetAsync(e.d_done, 0, DONE_BYTES, S_(e)));
     // pinned poll area: [0] active instances, [1 + g] active tiles of K group g
     HIP_TRY(hipHostMalloc((void **)&e.h_nactive, sizeof(int32_t) * (1 + PASC_MAX_K),
                           hipHostMallocDefault));

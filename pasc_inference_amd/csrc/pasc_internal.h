@@ -70,12 +70,14 @@ struct InstState {
 };
 
 // Completed-slot counters of the fused step (single 1-D problems): slot j counts on sub-counter
-// j % DONE_SUB, the last arrival of each sub-counter on the top counter.  One counter of all
-// slots would serialise the atomics (~10 ns each on one address, 2048 slots per pass); each
-// counter sits on its own 128-byte line.
+// j % DONE_SUB (its group), the last arrival of each sub-counter on the top counter.  One
+// counter of all slots would serialise the atomics (~10 ns each on one address, 2048 slots per
+// pass); each counter sits on its own 128-byte line.  The group partials (DONE_SUB x NPART
+// doubles) follow the counters in the same allocation.
 constexpr int DONE_SUB = 64;
 constexpr int DONE_STRIDE = 32;  // int32 words per counter
 constexpr int DONE_WORDS = (DONE_SUB + 1) * DONE_STRIDE;
+constexpr size_t DONE_BYTES = DONE_WORDS * sizeof(int32_t) + DONE_SUB * NPART * sizeof(double);
 
 // Solver constants of the step (kernel arguments of the step kernel and of the fused pass).
 struct StepParams {
@@ -148,8 +150,8 @@ struct Engine {
     };
     std::vector<Group> groups;
     int32_t *d_nactive = nullptr;        // device counter of active instances
-    int32_t *d_done = nullptr;           // completed-slot counters of the fused step
-                                         // (DONE_WORDS words, single problems only)
+    int32_t *d_done = nullptr;           // completed-slot counters and group partials of the
+                                         // fused step (DONE_BYTES, single problems only)
     int32_t *h_nactive = nullptr;        // pinned mirror
     PassShape sh[2];                     // [0] FP32 phase of FP32-bulk, [1] every FP64 pass
     int grid_cap = 0;                    // persistent grid cap (0: 32 blocks per CU)

@@ -34,11 +34,13 @@ struct PassArgs {
     int32_t kkt_every;  // evaluate the KKT residual on passes with it % kkt_every == 0
     int32_t need_f;     // 0: f comes from the quadratic identity (only the first pass sums it)
     const void *b;      // explicit linear terms of the phase's type, indexed like gamma (or null)
-    // fused step (single problems): the workgroup that completes the instance's last partial
-    // slot reduces the slots and runs the SPG step (no separate step launch)
+    // fused step (single problems): the slots are reduced in DONE_SUB groups as they complete,
+    // and the workgroup that completes the last group reduces the group partials and runs the
+    // SPG step (no separate step launch)
     StepParams sp;
     int32_t *done;      // DONE_WORDS counters of completed slots in this pass (back to 0
-                        // after); null: the step kernel runs the step
+                        // after), then DONE_SUB group partials; null: the step kernel runs
+                        // the step
     int32_t *nactive;   // active instances (decremented when one stops)
     int32_t *dirty;     // set when an instance stops (the batch's tile scan rebuilds)
 };
@@ -482,30 +484,62 @@ __device__ __forceinline__ void pass_tile_1d(const TileCtx<S, C, K> &cx, int chu
     }
 }
 
-// The step of one instance, run by the workgroup that completed its last partial slot: the
-// slots are reduced in a fixed order and step_scalar advances the instance.
+// Group partials of the fused step: DONE_SUB x NPART doubles after the counters.
+__device__ __forceinline__ double *group_partials(int32_t *done) {
+    return reinterpret_cast<double *>(done + DONE_WORDS);
+}
+
+// The step of one instance, run by the workgroup that completed its last group: the group
+// partials are reduced in a fixed order and step_scalar advances the instance.
 template <int NT>
-__device__ __forceinline__ void fused_step(const InstDesc *desc, InstState *state,
-                                           const double *parts, const StepParams &sp,
-                                           int32_t *dirty, int32_t *nactive, int inst,
-                                           InstState *lds_state, double *lds_red) {
+__device__ __forceinline__ void fused_step(const InstDesc *desc, InstState *state, int32_t *done,
+                                           const StepParams &sp, int32_t *dirty, int32_t *nactive,
+                                           int inst, InstState *lds_state, double *lds_red) {
     const InstDesc &D = desc[inst];
-    const double *part = parts + (int64_t)uni(D.tile0) * NPART;
-    step_instance<NT>(state + inst, uni(D.ntiles), D.alpha * D.s_kkt, sp, dirty, lds_state,
-                      lds_red, [&](int i, int q) { return load_coherent(part + (int64_t)i * NPART + q); });
+    const int ns = uni(D.ntiles);
+    const double *gp = group_partials(done);
+    step_instance<NT>(state + inst, ns < DONE_SUB ? ns : DONE_SUB, D.alpha * D.s_kkt, sp, dirty,
+                      lds_state, lds_red,
+                      [&](int i, int q) { return load_coherent(gp + (int64_t)i * NPART + q); });
     if (threadIdx.x == 0 && !lds_state->active) atomicSub(nactive, 1);
 }
 
 // Counts one completed partial slot of a single problem (its values already written with
-// store_coherent and publish_wait): slot j on sub-counter j % DONE_SUB, the last arrival of a
-// sub-counter on the top counter.  True in the workgroup that completed the instance's last
-// slot; every counter is back to 0 afterwards.  Thread 0 only.
-__device__ __forceinline__ bool count_slot(int32_t *done, int slot, int ns) {
-    const int g = slot % DONE_SUB;
+// store_coherent and publish_wait) on the sub-counter of its group g = slot % DONE_SUB.  True
+// in the workgroup that completed the group; the counter is back to 0 afterwards.  Thread 0.
+__device__ __forceinline__ bool count_slot(int32_t *done, int g, int ns) {
     const int n_g = ns / DONE_SUB + (g < ns % DONE_SUB ? 1 : 0);
     int32_t *sub = done + g * DONE_STRIDE;
     if (atomicAdd(sub, 1) != n_g - 1) return false;
     __hip_atomic_store(sub, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+// Run by the whole workgroup that completed group g: its slots g, g + DONE_SUB, ... are reduced
+// (per thread in slot order, then across the workgroup) into group partial g, which is
+// published and counted on the top counter.  True in thread 0 of the workgroup that completed
+// the last group (the instance's step is then due); the top counter is back to 0 afterwards.
+// Each slot is reduced once, by the workgroup completing its group, so the pass's last
+// workgroup only reduces its own group and the DONE_SUB group partials.
+template <int NT>
+__device__ __forceinline__ bool reduce_group(int32_t *done, const double *part, int g, int ns,
+                                             double *lds_red) {
+    double v[NPART] = {0.0, 0.0, 0.0, 0.0, -1.0};  // residual -1: not evaluated
+    for (int j = g + (int)threadIdx.x * DONE_SUB; j < ns; j += NT * DONE_SUB) {
+        double w[NPART];
+#pragma unroll
+        for (int q = 0; q < NPART; q++) w[q] = load_coherent(part + (int64_t)j * NPART + q);
+#pragma unroll
+        for (int q = 0; q < NPART - 1; q++) v[q] += w[q];
+        const double m = w[NPART - 1];
+        v[NPART - 1] = (m > v[NPART - 1] || m != m) ? m : v[NPART - 1];
+    }
+    block_reduce_part<NT>(v, lds_red);
+    if (threadIdx.x != 0) return false;
+    double *o = group_partials(done) + g * NPART;
+#pragma unroll
+    for (int q = 0; q < NPART; q++) store_coherent(o + q, v[q]);
+    publish_wait();
     int32_t *top = done + DONE_SUB * DONE_STRIDE;
     const int n_top = ns < DONE_SUB ? ns : DONE_SUB;
     if (atomicAdd(top, 1) != n_top - 1) return false;
@@ -526,10 +560,14 @@ __global__ __launch_bounds__(NT) void spg_pass_1d(PassArgs A) {
     __shared__ double lds_red[NPART * (NT / WAVE)];
     __shared__ C lds_x[2 * K * NT];
     __shared__ __attribute__((aligned(16))) S stage[USE_STAGE ? SG::ROWS * SG::ROW : 1];
-    __shared__ int32_t finished;  // instance whose last slot this workgroup wrote, or -1
+    __shared__ uint64_t groups_done;  // fused step: bit g set if this workgroup completed group g
+    __shared__ int32_t step_due;      // fused step: this workgroup completed the last group
     __shared__ InstState lds_state;  // fused step
     constexpr int TILE = tile_steps<NT, CH>();
-    if (threadIdx.x == 0) finished = -1;
+    if (threadIdx.x == 0) {
+        groups_done = 0;
+        step_due = 0;
+    }
     const int total = A.grp.off[A.grp.n];
     for (int j = blockIdx.x; j < total; j += gridDim.x) {
         int inst, slot;
@@ -580,30 +618,42 @@ __global__ __launch_bounds__(NT) void spg_pass_1d(PassArgs A) {
         if (threadIdx.x == 0) {
             double *o = A.part + (int64_t)slot_out * NPART;
             if (A.done) {
-                // Single problems: publish the slot, then count it; the workgroup that
-                // completes the instance runs its step after its last tile (the reduction
-                // order over the slots is fixed, so the result does not depend on which
-                // workgroup that is).  The slot is written through to the device coherence
-                // point and the count is taken once those stores completed; an agent-scope
-                // release fence would also write back every dirty line of this XCD's L2 (the
-                // gamma rows in flight), which costs a sixth of the pass.
+                // Single problems: publish the slot, then count it on its group's counter (the
+                // group is reduced after the tile loop).  The slot is written through to the
+                // device coherence point and counted once those stores completed; an
+                // agent-scope release fence would also write back every dirty line of this
+                // XCD's L2 (the gamma rows in flight), which costs a sixth of the pass.
+                const int g = slot % DONE_SUB;
 #pragma unroll
                 for (int i = 0; i < NPART; i++) store_coherent(o + i, acc[i]);
                 publish_wait();
-                if (count_slot(A.done, slot, uni(A.desc[inst].ntiles))) finished = inst;
+                if (count_slot(A.done, g, uni(A.desc[inst].ntiles))) groups_done |= 1ull << g;
             } else {
 #pragma unroll
                 for (int i = 0; i < NPART; i++) o[i] = acc[i];
             }
         }
     }
-    // after the tile loop, so that the step's registers do not add to the loop's
+    // After the tile loop, so that these registers do not add to the loop's: the groups this
+    // workgroup completed are reduced, and the workgroup that completes the last group runs the
+    // step.  Every reduction order is fixed by the slot tiling, so the result does not depend on
+    // which workgroups these are.  The fused step serves single problems (instance 0).
     if (A.done) {
         __syncthreads();
-        const int fin = finished;
-        if (fin >= 0)
-            fused_step<NT>(A.desc, A.state, A.part, A.sp, A.dirty, A.nactive, fin, &lds_state,
-                           lds_red);
+        uint64_t gmask = (uint64_t)uni((int64_t)groups_done);
+        if (gmask) {
+            const int ns = uni(A.desc[0].ntiles);
+            const double *part0 = A.part + (int64_t)uni(A.desc[0].tile0) * NPART;
+            while (gmask) {
+                const int g = __builtin_ctzll(gmask);
+                gmask &= gmask - 1;
+                if (reduce_group<NT>(A.done, part0, g, ns, lds_red)) step_due = 1;
+            }
+            __syncthreads();
+            if (step_due)
+                fused_step<NT>(A.desc, A.state, A.done, A.sp, A.dirty, A.nactive, 0, &lds_state,
+                               lds_red);
+        }
     }
 }
 

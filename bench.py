@@ -50,6 +50,8 @@ def parse():
                          "precision plus a copy kernel (for rocprofv3 traces and counters)")
     ap.add_argument("--scan-profile", action="store_true",
                     help="run only the batched scan (for rocprofv3 traces of the batched engine)")
+    ap.add_argument("--graph-profile", action="store_true",
+                    help="run only the image-graph solve (for rocprofv3 traces of the graph kernels)")
     ap.add_argument("--tune-cache", default=None,
                     help="autotuner cache file: launch shapes recorded there are reused, new ones appended")
     ap.add_argument("--verbose", action="store_true")
@@ -352,6 +354,11 @@ def main():
         from pasc_inference_amd.solver import Context
         with Context(0) as ctx:
             print(json.dumps(run_scan(args, ctx)))
+        return
+    if args.graph_profile:
+        from pasc_inference_amd.solver import Context
+        with Context(0) as ctx:
+            print(json.dumps(run_graph_c5(args, ctx)))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path "

@@ -677,6 +677,20 @@ struct GridArgs {
     GraphGeo geo;
 };
 
+// XCD-aware tile order of the graph kernels.  gfx950 dispatches workgroup w to XCD w % 8, and
+// each XCD has its own L2.  A grid node's neighbours one image row up or down lie a few tiles
+// away, so with tile j on workgroup j those rows were fetched by other XCDs' L2s.  Tile j of the
+// grid-stride loop is remapped so that the tiles with j % 8 == x (all on XCD x when the grid
+// is a multiple of 8) form the x-th contiguous eighth of [0, total): a band of image rows per
+// XCD.  A bijection of [0, total) for any total; the partial slots and their order of summation
+// do not change.
+constexpr int XCDS = 8;
+__device__ __forceinline__ int xcd_tile(int j, int total) {
+    const int q = total / XCDS, rem = total % XCDS;
+    const int x = j % XCDS, i = j / XCDS;
+    return x * q + (x < rem ? x : rem) + i;
+}
+
 template <typename S, typename C, int K>
 __device__ __forceinline__ void grid_gamma_at(const GridArgs &A, const InstDesc &D,
                                               const InstState &st, const S *src, int64_t r,
@@ -726,7 +740,7 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_dir(GridArgs A) {
     const bool is_f32 = sizeof(S) == 4;
     for (int j = blockIdx.x; j < total; j += gridDim.x) {
         int inst, slot;
-        find_tile(A.grp, j, inst, slot);
+        find_tile(A.grp, xcd_tile(j, total), inst, slot);
         const InstDesc &D = A.desc[inst];
         const InstState &st = A.state[inst];
         const int64_t N = D.T;  // nodes T*R
@@ -798,7 +812,7 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_ad(GridArgs A) {
     const int total = A.grp.off[A.grp.n];
     for (int j = blockIdx.x; j < total; j += gridDim.x) {
         int inst, slot;
-        find_tile(A.grp, j, inst, slot);
+        find_tile(A.grp, xcd_tile(j, total), inst, slot);
         const InstDesc &D = A.desc[inst];
         const int64_t N = D.T;
         // d_i was written by spg_pass_grid_dir into the buffer opposite to the parity

@@ -11,10 +11,20 @@
 
 namespace pasc {
 
+template <int K, bool IMG>
+static void launch_kg(const GridArgs &a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((spg_pass_grid_dir<PASS_S, PASS_S, K, 256, IMG>), dim3(grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((spg_pass_grid_ad<PASS_S, PASS_S, K, 256, IMG>), dim3(grid), dim3(256), 0, s, a);
+}
+
+// images (T == 1 on an implicit 4-neighbour grid) take the 32-bit neighbour walk
 template <int K>
 static void launch_k(const GridArgs &a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((spg_pass_grid_dir<PASS_S, PASS_S, K, 256>), dim3(grid), dim3(256), 0, s, a);
-    hipLaunchKernelGGL((spg_pass_grid_ad<PASS_S, PASS_S, K, 256>), dim3(grid), dim3(256), 0, s, a);
+    const bool img = a.geo.T == 1 && a.geo.width > 0 && !a.geo.ptr && a.geo.R < (int64_t(1) << 31);
+    if (img)
+        launch_kg<K, true>(a, grid, s);
+    else
+        launch_kg<K, false>(a, grid, s);
 }
 
 int GRID_FN(int K, const GridArgs &a, int grid, hipStream_t s) {

@@ -691,6 +691,33 @@ __device__ __forceinline__ int xcd_tile(int j, int total) {
     return x * q + (x < rem ? x : rem) + i;
 }
 
+// Neighbour walks of the graph kernels.  IMG: an image (T == 1, implicit 4-neighbour grid,
+// fewer than 2^31 nodes) in 32-bit index math, visiting the neighbours in for_spatial's order,
+// so that every sum is bitwise the same as through the general walk.
+template <bool IMG, typename F>
+__device__ __forceinline__ void grid_neighbours(const GraphGeo &G, int64_t n, F &&f) {
+    if constexpr (IMG) {
+        const uint32_t r = (uint32_t)n, W = (uint32_t)G.width, col = r % W;
+        if (r >= W) f(n - W, 1.0);
+        if (col > 0) f(n - 1, 1.0);
+        if (col < W - 1) f(n + 1, 1.0);
+        if (r + W < (uint32_t)G.R) f(n + W, 1.0);
+    } else {
+        for_neighbours(G, n, f);
+    }
+}
+
+template <bool IMG, typename F>
+__device__ __forceinline__ void grid_forward_edges(const GraphGeo &G, int64_t n, F &&f) {
+    if constexpr (IMG) {
+        const uint32_t r = (uint32_t)n, W = (uint32_t)G.width, col = r % W;
+        if (col < W - 1) f(n + 1, 1.0);
+        if (r + W < (uint32_t)G.R) f(n + W, 1.0);
+    } else {
+        for_forward_edges(G, n, f);
+    }
+}
+
 template <typename S, typename C, int K>
 __device__ __forceinline__ void grid_gamma_at(const GridArgs &A, const InstDesc &D,
                                               const InstState &st, const S *src, int64_t r,
@@ -733,7 +760,7 @@ __device__ __forceinline__ void grid_dataterm(const InstDesc &D, const void *xv,
     for (int k = 0; k < K; k++) q[k] *= C(D.bscale);
 }
 
-template <typename S, typename C, int K, int NT>
+template <typename S, typename C, int K, int NT, bool IMG>
 __global__ __launch_bounds__(NT) void spg_pass_grid_dir(GridArgs A) {
     __shared__ double lds[NPART * (NT / WAVE)];
     const int total = A.grp.off[A.grp.n];
@@ -761,7 +788,7 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_dir(GridArgs A) {
             C lap[K];
 #pragma unroll
             for (int k = 0; k < K; k++) lap[k] = C(0);
-            for_neighbours(A.geo, r, [&](int64_t n2, double w) {
+            grid_neighbours<IMG>(A.geo, r, [&](int64_t n2, double w) {
                 C gn[K];
                 grid_gamma_at<S, C, K>(A, D, st, src, n2, gn);
 #pragma unroll
@@ -806,7 +833,7 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_dir(GridArgs A) {
     }
 }
 
-template <typename S, typename C, int K, int NT>
+template <typename S, typename C, int K, int NT, bool IMG>
 __global__ __launch_bounds__(NT) void spg_pass_grid_ad(GridArgs A) {
     __shared__ double lds[NPART * (NT / WAVE)];
     const int total = A.grp.off[A.grp.n];
@@ -828,7 +855,7 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_ad(GridArgs A) {
                 const double d = (double)dr[k * D.ld + r];
                 acc[0] += d * d;
                 // d'A d = alpha * sum over the graph's edges of w (d_a - d_b)^2, each edge once
-                for_forward_edges(A.geo, r, [&](int64_t n2, double w) {
+                grid_forward_edges<IMG>(A.geo, r, [&](int64_t n2, double w) {
                     const double e = d - (double)dr[k * D.ld + n2];
                     acc[1] += alpha * w * e * e;
                 });

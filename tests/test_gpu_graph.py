@@ -119,3 +119,35 @@ def test_graph_validation(ctx):
     for g in bad:
         with pytest.raises(capi.PascError):
             Problem(ctx, x, 2, T=4, graph=g)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec_name", ["FP64", "FP32_BULK"])
+def test_image_walk_matches_general_walk(ctx, prec_name):
+    """Images (T = 1 on a 4-neighbour grid) run the graph kernels with a 32-bit neighbour walk;
+    the same image given as an explicit CSR adjacency (up, left, right, down, the general walk's
+    order) runs the general kernels.  The iterates agree bit for bit, so the image walk inherits
+    the general walk's parity with the oracle."""
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Problem
+    prec = getattr(capi, prec_name)
+    w, h, K, xdim = 37, 23, 4, 3
+    rng = np.random.default_rng(17)
+    lab = rng.integers(0, K, size=(h, w))
+    lab[5:15, 8:30] = 1
+    col = rng.uniform(0, 1, size=(K, xdim))
+    x = (col[lab.ravel()] + rng.normal(0, 0.2, size=(w * h, xdim))).ravel()
+    outs = []
+    for graph in (None, grid_csr(w, h)):
+        if graph is None:
+            P = Problem(ctx, x, K, T=1, xdim=xdim, grid_width=w, precision=prec, epssqr=3.0)
+        else:
+            P = Problem(ctx, x, K, T=1, xdim=xdim, graph=graph, precision=prec, epssqr=3.0)
+        P.set_theta(col.ravel().copy())
+        P.set_gamma_uniform()
+        r = P.gamma_solve(maxit=150, stop_mask=0, polish_maxit=30)
+        outs.append((r.it, r.fx, P.get_gamma()))
+        P.close()
+    assert outs[0][0] == outs[1][0] >= 150
+    assert outs[0][1] == outs[1][1]
+    assert np.array_equal(outs[0][2], outs[1][2])

@@ -383,7 +383,8 @@ static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<in
     hipStream_t s = S_(e);
     // single 1-D problems run the step inside the pass.  (Run in the graph path's dAd kernel,
     // the step's serial tail cost ~8 us per iteration against ~6 us for the separate step and
-    // scan launches under graph replay, so the graph kernels keep the step kernel.)
+    // scan launches under graph replay, so the graph kernels keep the step kernel; single graph
+    // problems launch no scan.)
     const bool fused = e.n_inst == 1 && e.stencil == STENCIL_TIME;
     if (ev) {
         hipEvent_t a;
@@ -427,8 +428,16 @@ static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<in
         HIP_TRY(hipGetLastError());
         return PASC_OK;
     }
+    // A single graph problem: the step kernel counts the instance out when it stops, and the
+    // graph kernels skip the tiles of a stopped instance, so no scan is launched.
+    const bool single = e.n_inst == 1;
     hipLaunchKernelGGL(spg_step, dim3(std::min(e.n_inst, std::max(e.ctx->n_cu, 1) * 4)), dim3(256),
-                       0, s, e.d_desc, e.d_state, e.d_part, e.n_inst, ph.sp, e.d_nactive + 1);
+                       0, s, e.d_desc, e.d_state, e.d_part, e.n_inst, ph.sp, e.d_nactive + 1,
+                       single ? e.d_nactive : nullptr);
+    if (single) {
+        HIP_TRY(hipGetLastError());
+        return PASC_OK;
+    }
     ScanArgs sa{};
     sa.n_groups = (int)e.groups.size();
     for (size_t gi = 0; gi < e.groups.size(); gi++)

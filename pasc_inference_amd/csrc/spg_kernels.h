@@ -768,6 +768,8 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_dir(GridArgs A) {
     for (int j = blockIdx.x; j < total; j += gridDim.x) {
         int inst, slot;
         find_tile(A.grp, xcd_tile(j, total), inst, slot);
+        // a stopped single problem keeps its tiles (no scan runs): they are skipped
+        if (uni(A.state[inst].active) == 0) continue;
         const InstDesc &D = A.desc[inst];
         const InstState &st = A.state[inst];
         const int64_t N = D.T;  // nodes T*R
@@ -840,6 +842,7 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_ad(GridArgs A) {
     for (int j = blockIdx.x; j < total; j += gridDim.x) {
         int inst, slot;
         find_tile(A.grp, xcd_tile(j, total), inst, slot);
+        if (uni(A.state[inst].active) == 0) continue;
         const InstDesc &D = A.desc[inst];
         const int64_t N = D.T;
         // d_i was written by spg_pass_grid_dir into the buffer opposite to the parity

@@ -8,9 +8,11 @@ namespace pasc {
 
 // Step kernel: workgroups stride over the instances; each reduces one active instance's partial
 // slots in a fixed order (deterministic) and runs step_scalar.
+// nactive: decremented when an instance stops (single problems, which launch no scan); null
+// when the active-tile scan recounts the active instances.
 __global__ __launch_bounds__(256) void spg_step(const InstDesc *desc, InstState *state,
                                                  const double *part, int n_inst, StepParams P,
-                                                 int32_t *dirty) {
+                                                 int32_t *dirty, int32_t *nactive) {
     // one workgroup per instance: a single instance of T = 1e7 owns ~1e4 partial slots
     __shared__ double lds[NPART * 4];
     __shared__ InstState lds_state;
@@ -20,6 +22,7 @@ __global__ __launch_bounds__(256) void spg_step(const InstDesc *desc, InstState 
         const double *pw = part + (int64_t)uni(D.tile0) * NPART;
         step_instance<256>(state + w, uni(D.ntiles), D.alpha * D.s_kkt, P, dirty, &lds_state, lds,
                            [&](int i, int q) { return pw[(int64_t)i * NPART + q]; });
+        if (nactive && threadIdx.x == 0 && !lds_state.active) atomicSub(nactive, 1);
         __syncthreads();  // lds_state is reused by the next instance
     }
 }

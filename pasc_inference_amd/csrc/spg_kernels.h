@@ -734,30 +734,63 @@ __device__ __forceinline__ void grid_gamma_at(const GridArgs &A, const InstDesc 
     }
 }
 
+// gamma_i at node r from per-slot values held in registers (as grid_gamma_at); FIRST: the
+// pass starts from an unprojected gamma_0
+template <typename S, typename C, int K, bool FIRST>
+__device__ __forceinline__ void grid_gamma_u(const S *src, const S *dprev, C beta, int64_t ld,
+                                             int64_t r, C (&v)[K]) {
+    if constexpr (FIRST) {
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] = C(src[k * ld + r]);
+        project_simplex<K, C>(v);
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] = C(src[k * ld + r]) + beta * C(dprev[k * ld + r]);
+    }
+}
+
+// The data-term fields of one instance in scalar registers (read once per slot), and the data
+// term q_k = bscale * ||x_r - theta_k||^2 of node r from them (0 at held-out nodes; -b for an
+// explicit linear term).
+struct GridData {
+    const uint8_t *heldout;
+    const double *b64;
+    const float *b32;
+    const double *theta;
+    int64_t xoff, ld;
+    int32_t xdim;
+    double bscale;
+};
+
+__device__ __forceinline__ GridData grid_data(const InstDesc &D) {
+    return GridData{uni(D.heldout), uni(D.b64), uni(D.b32), D.theta,
+                    uni(D.xoff), uni(D.ld), uni(D.xdim), uni(D.bscale)};
+}
+
 template <typename C, int K>
-__device__ __forceinline__ void grid_dataterm(const InstDesc &D, const void *xv, int64_t r,
-                                              bool is_f32, C (&q)[K]) {
-    if (D.b64) {  // explicit linear term: q = -b
+__device__ __forceinline__ void grid_dataterm_u(const GridData &G, const void *xv, int64_t r,
+                                                bool is_f32, C (&q)[K]) {
+    if (G.b64) {  // explicit linear term: q = -b
 #pragma unroll
         for (int k = 0; k < K; k++)
-            q[k] = is_f32 ? -C(D.b32[k * D.ld + r]) : -C(D.b64[k * D.ld + r]);
+            q[k] = is_f32 ? -C(G.b32[k * G.ld + r]) : -C(G.b64[k * G.ld + r]);
         return;
     }
-    const bool obs = !(D.heldout && D.heldout[r]);
+    const bool obs = !(G.heldout && G.heldout[r]);
 #pragma unroll
     for (int k = 0; k < K; k++) q[k] = C(0);
     if (!obs) return;
-    for (int c = 0; c < D.xdim; c++) {
-        const C xc = is_f32 ? C(reinterpret_cast<const float *>(xv)[D.xoff + c * D.ld + r])
-                            : C(reinterpret_cast<const double *>(xv)[D.xoff + c * D.ld + r]);
+    for (int c = 0; c < G.xdim; c++) {
+        const C xc = is_f32 ? C(reinterpret_cast<const float *>(xv)[G.xoff + c * G.ld + r])
+                            : C(reinterpret_cast<const double *>(xv)[G.xoff + c * G.ld + r]);
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            const C dx = xc - C(D.theta[k * D.xdim + c]);
+            const C dx = xc - C(G.theta[k * G.xdim + c]);
             q[k] += dx * dx;
         }
     }
 #pragma unroll
-    for (int k = 0; k < K; k++) q[k] *= C(D.bscale);
+    for (int k = 0; k < K; k++) q[k] *= C(G.bscale);
 }
 
 template <typename S, typename C, int K, int NT, bool IMG>
@@ -772,34 +805,69 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_dir(GridArgs A) {
         if (uni(A.state[inst].active) == 0) continue;
         const InstDesc &D = A.desc[inst];
         const InstState &st = A.state[inst];
-        const int64_t N = D.T;  // nodes T*R
-        const int par = st.parity;
+        // per-slot fields in scalar registers (read once, not after every store)
+        const int64_t N = uni(D.T);  // nodes T*R
+        const int64_t ld = uni(D.ld), goff = uni(D.goff);
+        const int msub = uni(D.msub);
+        const int par = uni(st.parity);
+        const bool first = uni(st.first) != 0;
+        const C beta = C(uni(st.beta_apply));
+        const C alpha = C(uni(D.alpha)), a_dir = C(uni(st.alpha_dir)), s = C(uni(D.s_kkt));
+        const GridData gd = grid_data(D);
         // the KKT residual on every pass: these kernels are bound by their neighbour loads, the
         // extra projection costs nothing measurable, and the solve stops at the first pass that
         // reaches the target (the residual is not monotone along SPG)
         double acc[NPART] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      for (int m = 0; m < D.msub; m++) {
-        const int64_t r = (int64_t)(slot * D.msub + m) * NT + threadIdx.x;
-        if ((int64_t)(slot * D.msub + m) * NT >= N) break;
-        const S *src = reinterpret_cast<const S *>(par ? A.gamma1 : A.gamma0) + D.goff;
-        S *dst = reinterpret_cast<S *>(par ? A.gamma0 : A.gamma1) + D.goff;
-        S *dr = reinterpret_cast<S *>(par ? A.dir0 : A.dir1) + D.goff;
+      for (int m = 0; m < msub; m++) {
+        const int64_t r = (int64_t)(slot * msub + m) * NT + threadIdx.x;
+        if ((int64_t)(slot * msub + m) * NT >= N) break;
+        const S *src = reinterpret_cast<const S *>(par ? A.gamma1 : A.gamma0) + goff;
+        S *dst = reinterpret_cast<S *>(par ? A.gamma0 : A.gamma1) + goff;
+        S *dr = reinterpret_cast<S *>(par ? A.dir0 : A.dir1) + goff;
         if (r < N) {
             C g0[K];
-            grid_gamma_at<S, C, K>(A, D, st, src, r, g0);
             C lap[K];
 #pragma unroll
             for (int k = 0; k < K; k++) lap[k] = C(0);
-            grid_neighbours<IMG>(A.geo, r, [&](int64_t n2, double w) {
-                C gn[K];
-                grid_gamma_at<S, C, K>(A, D, st, src, n2, gn);
+            if constexpr (IMG) {
+                // The walk's order (up, left, right, down) without branches: a missing
+                // neighbour reads the node itself and its term is dropped by a select, so the
+                // sums are bitwise the general walk's and all loads are issued before any use.
+                const uint32_t ru = (uint32_t)r, W = (uint32_t)A.geo.width, col = ru % W;
+                const bool has[4] = {ru >= W, col > 0, col < W - 1, ru + W < (uint32_t)A.geo.R};
+                const int64_t nb[4] = {has[0] ? r - (int64_t)W : r, has[1] ? r - 1 : r,
+                                       has[2] ? r + 1 : r, has[3] ? r + (int64_t)W : r};
+                const S *dprev = reinterpret_cast<const S *>(par ? A.dir1 : A.dir0) + goff;
+                // one branch-free block per case, so that the five nodes' loads issue together
+                auto gather = [&](auto first_case) {
+                    constexpr bool FIRST = decltype(first_case)::value;
+                    C gn[4][K];
+                    grid_gamma_u<S, C, K, FIRST>(src, dprev, beta, ld, r, g0);
 #pragma unroll
-                for (int k = 0; k < K; k++) lap[k] += C(w) * (g0[k] - gn[k]);
-            });
+                    for (int q = 0; q < 4; q++)
+                        grid_gamma_u<S, C, K, FIRST>(src, dprev, beta, ld, nb[q], gn[q]);
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+#pragma unroll
+                        for (int k = 0; k < K; k++)
+                            lap[k] = has[q] ? lap[k] + C(1.0) * (g0[k] - gn[q][k]) : lap[k];
+                };
+                if (first)
+                    gather(std::true_type{});
+                else
+                    gather(std::false_type{});
+            } else {
+                grid_gamma_at<S, C, K>(A, D, st, src, r, g0);
+                grid_neighbours<IMG>(A.geo, r, [&](int64_t n2, double w) {
+                    C gn[K];
+                    grid_gamma_at<S, C, K>(A, D, st, src, n2, gn);
+#pragma unroll
+                    for (int k = 0; k < K; k++) lap[k] += C(w) * (g0[k] - gn[k]);
+                });
+            }
             C q[K];
-            grid_dataterm<C, K>(D, A.x, r, is_f32, q);
+            grid_dataterm_u<C, K>(gd, A.x, r, is_f32, q);
             C y[K], p[K], z[K], g[K];
-            const C alpha = C(D.alpha), a_dir = C(st.alpha_dir), s = C(D.s_kkt);
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 g[k] = alpha * lap[k] + q[k];
@@ -813,8 +881,8 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_dir(GridArgs A) {
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const C d = p[k] - g0[k];
-                dst[k * D.ld + r] = S(g0[k]);
-                dr[k * D.ld + r] = S(d);
+                dst[k * ld + r] = S(g0[k]);
+                dr[k * ld + r] = S(d);
                 ddt += (double)d * (double)d;
                 if (y[k] <= tau) off += (double)(tau - y[k]) * (double)g0[k];
                 acc[3] += 0.5 * ((double)g[k] + (double)q[k]) * (double)g0[k];
@@ -844,24 +912,44 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_ad(GridArgs A) {
         find_tile(A.grp, xcd_tile(j, total), inst, slot);
         if (uni(A.state[inst].active) == 0) continue;
         const InstDesc &D = A.desc[inst];
-        const int64_t N = D.T;
+        const int64_t N = uni(D.T);
+        const int64_t ld = uni(D.ld);
+        const int msub = uni(D.msub);
+        const double alpha = uni(D.alpha);
         // d_i was written by spg_pass_grid_dir into the buffer opposite to the parity
-        const S *dr = reinterpret_cast<const S *>(A.state[inst].parity ? A.dir0 : A.dir1) + D.goff;
+        const S *dr = reinterpret_cast<const S *>(uni(A.state[inst].parity) ? A.dir0 : A.dir1) +
+                      uni(D.goff);
         double acc[NPART] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      for (int m = 0; m < D.msub; m++) {
-        const int64_t r = (int64_t)(slot * D.msub + m) * NT + threadIdx.x;
-        if ((int64_t)(slot * D.msub + m) * NT >= N) break;
+      for (int m = 0; m < msub; m++) {
+        const int64_t r = (int64_t)(slot * msub + m) * NT + threadIdx.x;
+        if ((int64_t)(slot * msub + m) * NT >= N) break;
         if (r < N) {
-            const double alpha = D.alpha;
+            if constexpr (IMG) {
+                // the forward edges (right, down) without branches, as in the direction kernel
+                const uint32_t ru = (uint32_t)r, W = (uint32_t)A.geo.width, col = ru % W;
+                const bool hr = col < W - 1, hd = ru + W < (uint32_t)A.geo.R;
+                const int64_t nr = hr ? r + 1 : r, nd = hd ? r + (int64_t)W : r;
+                const double w = 1.0;
 #pragma unroll
-            for (int k = 0; k < K; k++) {
-                const double d = (double)dr[k * D.ld + r];
-                acc[0] += d * d;
-                // d'A d = alpha * sum over the graph's edges of w (d_a - d_b)^2, each edge once
-                grid_forward_edges<IMG>(A.geo, r, [&](int64_t n2, double w) {
-                    const double e = d - (double)dr[k * D.ld + n2];
-                    acc[1] += alpha * w * e * e;
-                });
+                for (int k = 0; k < K; k++) {
+                    const double d = (double)dr[k * ld + r];
+                    const double e1 = d - (double)dr[k * ld + nr];
+                    const double e2 = d - (double)dr[k * ld + nd];
+                    acc[0] += d * d;
+                    acc[1] = hr ? acc[1] + alpha * w * e1 * e1 : acc[1];
+                    acc[1] = hd ? acc[1] + alpha * w * e2 * e2 : acc[1];
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const double d = (double)dr[k * ld + r];
+                    acc[0] += d * d;
+                    // d'A d = alpha * sum over the graph's edges of w (d_a - d_b)^2, each edge once
+                    grid_forward_edges<IMG>(A.geo, r, [&](int64_t n2, double w) {
+                        const double e = d - (double)dr[k * ld + n2];
+                        acc[1] += alpha * w * e * e;
+                    });
+                }
             }
         }
       }

@@ -1030,8 +1030,9 @@ int engine_set_shape(Engine &e, const PassShape &s32, const PassShape &s64) {
 
 // Self-written autotuner: times the fused pass of this problem (or batch) for every compiled
 // (threads per block, time steps per thread, prefetch) triple, and with prefetch also for
-// 2048 / 1024 / 512 partial slots per instance and for at least 4 tiles per slot (more tiles
-// per workgroup lengthen its pipeline) -- 20 passes each from the current iterate with the KKT
+// 8192 / 4096 / 2048 / 1024 / 512 partial slots per instance (more slots put more one-wave
+// workgroups on each SIMD) and for at least 4 tiles per slot (more tiles per workgroup lengthen
+// its pipeline) -- 20 passes each from the current iterate with the KKT
 // residual on every 4th pass, hipEvents around every launch.  An FP32-bulk problem runs 20 FP32
 // and 20 FP64 polish passes per candidate, and the fastest shape of each phase is kept.  The
 // iterate is restored afterwards.
@@ -1045,7 +1046,8 @@ int engine_autotune(Engine &e, PassShape *s32, PassShape *s64, double us[2]) {
     const bool f32 = e.precision == PASC_FP32_BULK;
     const int chunks32[2] = {4, 8}, chunks64[2] = {2, 4};
     const int blocks[4] = {64, 128, 256, 512};
-    const int slot_opts[4][2] = {{2048, 1}, {1024, 1}, {512, 1}, {2048, 4}};
+    constexpr int NSLOT = 6;
+    const int slot_opts[NSLOT][2] = {{2048, 1}, {1024, 1}, {512, 1}, {2048, 4}, {4096, 1}, {8192, 1}};
     pasc_spg_params p;
     pasc_spg_default_params(&p);
     p.maxit = 20;
@@ -1057,7 +1059,7 @@ int engine_autotune(Engine &e, PassShape *s32, PassShape *s64, double us[2]) {
     double best[2] = {1e300, 1e300};
     PassShape bs[2] = {e.sh[0], e.sh[1]};
     for (int pi = 0; pi < 2; pi++)
-    for (int si = 0; si < (pi ? 4 : 1); si++)
+    for (int si = 0; si < (pi ? NSLOT : 1); si++)
     for (int ci = 0; ci < 2; ci++)
     for (int bi = 0; bi < 4; bi++) {
         const PassShape c32{blocks[bi], chunks32[ci], pi, slot_opts[si][0], slot_opts[si][1]};

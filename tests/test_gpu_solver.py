@@ -347,6 +347,35 @@ def test_one_wave_workgroups_follow_oracle(ctx, oracle, prec_name, chunk):
     P.close()
 
 
+@pytest.mark.parametrize("block", [64, 256])
+def test_many_slots_per_group_follow_oracle(ctx, oracle, block):
+    """Up to 8192 partial slots at T = 6e5.  With 64 threads x 2 steps the tiles hold 120 steps
+    and every tile is a slot (5000): the fused step's 64 completion groups then hold 78-79 slots,
+    more than the one-wave workgroup's 64 threads, so the group reduction loops.  With 256
+    threads there are 1191 slots.  The first iterations follow the oracle's to rounding."""
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Problem
+    T, K, eps = 600000, 3, 30.0
+    levels = [0.0, 1.0, 3.0]
+    x, _ = _signal(43, T, levels, 1e-3, 0.7)
+    theta = np.array(levels)
+    b = oracle.assemble_b(x, theta, T, 1, K)
+    P = Problem(ctx, x, K, precision=capi.FP64, epssqr=eps)
+    P.set_theta(theta)
+    P.set_launch_shape(block, 2, 1, 8192, 1)
+    for maxit in (1, 3, 20):
+        xs, rr = oracle.spg_solve(np.full(T * K, 1.0 / K), b, T, 1, K, eps / T, maxit=maxit,
+                                  stop_difff=0, nthreads=8)
+        P.set_gamma_uniform()
+        r = P.gamma_solve(maxit=maxit, stop_mask=0)
+        err = np.max(np.abs(P.get_gamma() - xs))
+        print(f"block {block} maxit {maxit}: err {err:.2e} f {r.fx!r} oracle {rr.fx!r}", flush=True)
+        assert r.it == maxit
+        assert err <= 1e-10
+        assert abs(r.fx - rr.fx) <= 1e-12 * abs(rr.fx)
+    P.close()
+
+
 def test_autotune_file_cache(tmp_path):
     """A second context finds the shape the first one tuned in the cache file (no re-tuning)."""
     import json

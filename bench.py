@@ -1,9 +1,10 @@
 """Benchmark of the FEM-H1 K-means gamma step on one MI355X.
 
 Metric: time-to-KKT 1e-6 of one gamma QP at T=1e7, K=3 (levels {0,1,3}, switch 1e-4, noise
-sigma 1, seed 1, epsilon 100, theta fixed at the levels, gamma0 = 1/K), FP32-bulk path with FP64
-reductions and FP64-certified KKT; plus the batched epsilon-scan throughput (256 uneven series x
-16 epsilons) against the plain FP64 per-solve path on the same GPU.
+sigma 1, seed 1, epsilon 100, theta fixed at the levels, gamma0 = 1/K): the FP64 path and the
+FP32-bulk path (FP64 reductions, FP64-certified KKT) are both timed with autotuned launch shapes,
+and `value` is the faster of the two (`path` names it); plus the batched epsilon-scan throughput
+(256 uneven series x 16 epsilons) against the plain FP64 per-solve path on the same GPU.
 
 A "step" is one complete gamma solve from gamma0 to KKT <= 1e-6 with the data resident in HBM.
 Prints one JSON line.

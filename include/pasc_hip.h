@@ -137,21 +137,29 @@ typedef struct {
     int32_t passes_fp64;
 } pasc_spg_result;
 
-/* Outer alternation settings (tssolver.h:13-15) */
+/* Outer alternation settings (tssolver.h:13-15, TSSolver(tsdata, annealing) :81) */
 typedef struct {
     int32_t maxit;             /* default 300 */
     double eps;                /* default 1e-6, |L - L_old| < eps */
     int32_t theta_fixed;       /* TSSolver::set_solution_theta: theta is never updated */
-    int32_t skip_first_theta;  /* start with a gamma step from the given theta */
+    int32_t skip_first_theta;  /* start with a gamma step from the given theta (first run only) */
+    int32_t annealing;         /* TSSolver::set_annealing (tssolver.h:486-488): runs of the
+                                  alternation, default 1; each run after the first restarts from
+                                  a random gamma and the run of the smallest AIC is kept */
+    uint64_t restart_seed;     /* seed of the restarts' random gamma (pasc_generate_uniform);
+                                  default 13, the seed of GeneralVector::set_random */
 } pasc_ts_params;
 
 typedef struct {
-    int32_t it;            /* outer iterations performed */
-    int32_t spg_it_sum;    /* QPSolver::it_sum over the alternation */
-    double L;              /* final model value (GraphH1FEMModel::get_L = gamma solver fx) */
+    int32_t it;            /* TSSolver::it_last of the last run: the index of the iteration whose
+                              |L - L_old| < eps broke the loop (tssolver.h:642-650), or maxit */
+    int32_t spg_it_sum;    /* QPSolver::it_sum: SPG iterations of every gamma solve of the call */
+    double L;              /* model value of the kept run (GraphH1FEMModel::get_L = gamma fx) */
     double deltaL;
     int32_t increased;     /* times L increased ("ERROR: objective function increased") */
-    double aic;            /* GraphH1FEMModel::get_aic: 2 log L + K */
+    double aic;            /* GraphH1FEMModel::get_aic of the kept run: 2 log L + K */
+    int32_t it_sum;        /* TSSolver::it_sum: it_last summed over the runs */
+    int32_t best_run;      /* index of the kept run (0 without annealing) */
 } pasc_ts_result;
 
 /* ---- errors / version ---- */
@@ -330,6 +338,9 @@ int pasc_batch_set_launch_shape(pasc_batch *b, const pasc_launch_shape *shape);
 int pasc_generate_markov(uint64_t seed, int64_t T, int32_t K, const double *levels,
                          double p_switch, double sigma, double *x, int32_t *labels);
 int pasc_generate_mask(uint64_t seed, int64_t n, double fraction, uint8_t *mask);
+/* n uniforms in [0, 1) (SplitMix64 z, (z >> 11) * 2^-53): the random gamma of the annealing
+ * restarts, in place of GeneralVector<PetscVector>::set_random (generalvector.cpp:7-29) */
+int pasc_generate_uniform(uint64_t seed, int64_t n, double *out);
 int pasc_generate_rect_image(uint64_t seed, int32_t width, int32_t height, int32_t K,
                              int32_t n_rect, double sigma, double *x, double *colours,
                              int32_t *labels);

@@ -540,9 +540,20 @@ double or_objective(const double *x, const double *b, int64_t T, int64_t R, int 
 /* outer alternation                                                                          */
 /* ------------------------------------------------------------------------------------------ */
 
-static int or_ts_solve_fem(const double *x, double *gamma, double *theta, int64_t T, int K,
-                           int xdim, double epssqr, const uint8_t *heldout,
-                           const or_ts_params *tp, const or_spg_params *sp, or_ts_result *res);
+/* One run of the alternation (tssolver.h:542-650): the theta step (unless theta is fixed, or
+ * skipped on the first iteration), the gamma step, L = fx of the gamma solve (get_L), and the
+ * stop at the first |L - L_old| < eps.  The reference breaks before its it++, so it_last is the
+ * index of the breaking iteration, or maxit when the loop runs out. */
+typedef struct {
+    int it;        /* it_last */
+    int spg;       /* SPG iterations of every gamma solve of the run (QPSolver::it_sum) */
+    double L, deltaL;
+    int increased;
+} ts_run_out;
+
+static int ts_run_fem(const double *x, double *gamma, double *theta, int64_t T, int K, int xdim,
+                      double epssqr, const uint8_t *heldout, const or_ts_params *tp,
+                      const or_spg_params *sp, int skip_first, ts_run_out *o);
 
 int64_t or_fem_reduced_T(int64_t T1, double fem_reduce) {
     /* fem.cpp:198 / femhat.cpp:224: T_reduced = ceil(T * fem_reduce) when reduced */
@@ -603,27 +614,21 @@ void or_fem_prolongate(const double *in, double *out, int64_t T1, int64_t T2, in
         }
 }
 
-int or_ts_solve(const double *x, double *gamma, double *theta, int64_t T, int64_t R, int K,
-                int xdim, double epssqr, const int64_t *nbr_ptr, const int32_t *nbr_idx,
-                const uint8_t *heldout, const or_ts_params *tp, const or_spg_params *sp,
-                or_ts_result *res) {
-    const int fem = tp->fem_reduce > 0.0 && tp->fem_reduce < 1.0 && tp->fem_type > 0;
-    if (fem) {
-        if (R != 1) return -1;
-        return or_ts_solve_fem(x, gamma, theta, T, K, xdim, epssqr, heldout, tp, sp, res);
-    }
+static int ts_run_plain(const double *x, double *gamma, double *theta, int64_t T, int64_t R,
+                        int K, int xdim, double epssqr, const int64_t *nbr_ptr,
+                        const int32_t *nbr_idx, const uint8_t *heldout, const or_ts_params *tp,
+                        const or_spg_params *sp, int skip_first, ts_run_out *o) {
     const int nthreads = fix_threads(sp->nthreads);
     const int64_t n = T * R * K;
     double *b = (double *)malloc(sizeof(double) * (size_t)n);
     if (!b) return -1;
     /* graphh1fem.cpp:146-151 with no FEM reduction */
     const double alpha = tp->scalef ? epssqr / (double)T : epssqr;
-
     double L = DBL_MAX, L_old, deltaL = DBL_MAX;
     int it, spg_sum = 0, increased = 0;
     for (it = 0; it < tp->maxit; it++) {
         /* theta step (tssolver.h:548-560) */
-        if (!tp->theta_fixed && !(it == 0 && tp->skip_first_theta))
+        if (!tp->theta_fixed && !(it == 0 && skip_first))
             or_theta_update(x, gamma, theta, T, R, K, xdim, heldout, nthreads);
         /* gamma step (tssolver.h:584-596) */
         or_assemble_b(x, theta, b, T, R, K, xdim, tp->scalef, heldout, nthreads);
@@ -633,21 +638,97 @@ int or_ts_solve(const double *x, double *gamma, double *theta, int64_t T, int64_
             return -1;
         }
         spg_sum += r.it;
-        /* tssolver.h:628-644 */
+        /* tssolver.h:628-650 */
         L_old = L;
         L = r.fx;
         deltaL = fabs(L - L_old);
         if (L - L_old > 0) increased++;
-        if (deltaL < tp->eps) { it++; break; }
+        if (deltaL < tp->eps) break;
     }
-    if (res) {
-        res->it = it;
-        res->spg_it_sum = spg_sum;
-        res->L = L;
-        res->deltaL = deltaL;
-        res->increased = increased;
-    }
+    o->it = it;
+    o->spg = spg_sum;
+    o->L = L;
+    o->deltaL = deltaL;
+    o->increased = increased;
     free(b);
+    return 0;
+}
+
+void or_uniform(uint64_t seed, int64_t n, double *out) {
+    uint64_t s = seed;
+    for (int64_t i = 0; i < n; i++) {
+        s += 0x9E3779B97F4A7C15ull;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        out[i] = (double)(z >> 11) * 0x1.0p-53;
+    }
+}
+
+/* TSSolver::solve (tssolver.h:491-715): `annealing` runs of the alternation.  Each run ends with
+ * AIC = 2 log L + K (graphh1fem.h:443-447); with more than one run, the gamma and theta of the
+ * run with the smallest AIC are kept (tssolver.h:669-692, the temp vectors), and every run after
+ * the first starts from the random gamma of set_random (tssolver.h:687-689).  skip_first_theta
+ * applies to the first run only (the random restarts begin with a theta step, as in the
+ * reference). */
+int or_ts_solve(const double *x, double *gamma, double *theta, int64_t T, int64_t R, int K,
+                int xdim, double epssqr, const int64_t *nbr_ptr, const int32_t *nbr_idx,
+                const uint8_t *heldout, const or_ts_params *tp, const or_spg_params *sp,
+                or_ts_result *res) {
+    const int fem = tp->fem_reduce > 0.0 && tp->fem_reduce < 1.0 && tp->fem_type > 0;
+    if (fem && R != 1) return -1;
+    const int runs = tp->annealing > 1 ? tp->annealing : 1;
+    const int64_t n = T * R * K;
+    const int nth = K * xdim;
+    double *gbest = NULL, *tbest = NULL;
+    if (runs > 1) {
+        gbest = (double *)malloc(sizeof(double) * (size_t)n);
+        tbest = (double *)malloc(sizeof(double) * (size_t)nth);
+        if (!gbest || !tbest) { free(gbest); free(tbest); return -1; }
+    }
+    double best_aic = DBL_MAX, L_keep = DBL_MAX, dL_keep = DBL_MAX;
+    int it_last = 0, it_sum = 0, spg_sum = 0, increased = 0, best_run = 0;
+    for (int a = 0; a < runs; a++) {
+        ts_run_out o;
+        const int skip = a == 0 ? tp->skip_first_theta : 0;
+        const int rc = fem ? ts_run_fem(x, gamma, theta, T, K, xdim, epssqr, heldout, tp, sp, skip, &o)
+                           : ts_run_plain(x, gamma, theta, T, R, K, xdim, epssqr, nbr_ptr, nbr_idx,
+                                          heldout, tp, sp, skip, &o);
+        if (rc) { free(gbest); free(tbest); return -1; }
+        it_last = o.it;
+        it_sum += o.it;
+        spg_sum += o.spg;
+        increased += o.increased;
+        const double aic = 2.0 * log(o.L) + (double)K;
+        if (runs <= 1 || aic < best_aic) {
+            if (runs > 1) {
+                memcpy(gbest, gamma, sizeof(double) * (size_t)n);
+                memcpy(tbest, theta, sizeof(double) * (size_t)nth);
+            }
+            best_aic = aic;
+            L_keep = o.L;
+            dL_keep = o.deltaL;
+            best_run = a;
+        }
+        if (a < runs - 1) or_uniform(tp->restart_seed, n, gamma);
+    }
+    if (runs > 1) {
+        memcpy(gamma, gbest, sizeof(double) * (size_t)n);
+        memcpy(theta, tbest, sizeof(double) * (size_t)nth);
+    }
+    free(gbest);
+    free(tbest);
+    if (res) {
+        res->it = it_last;
+        res->spg_it_sum = spg_sum;
+        res->L = L_keep;
+        res->deltaL = dL_keep;
+        res->increased = increased;
+        res->aic = best_aic;
+        res->it_sum = it_sum;
+        res->best_run = best_run;
+    }
     return 0;
 }
 
@@ -692,9 +773,9 @@ double or_host_copy_bandwidth(int64_t nbytes, int reps, int nthreads) {
  * (:274-277), b = -reduced residual / T2 with scalef (:281-286), A = epssqr/T * A0 on the coarse
  * grid with scalef, epssqr*T2/T without (:146-151), the coarse solution is prolongated back
  * (:296-298); theta is updated from the fine gamma.  Held-out steps carry no residual. */
-static int or_ts_solve_fem(const double *x, double *gamma, double *theta, int64_t T, int K,
-                           int xdim, double epssqr, const uint8_t *heldout,
-                           const or_ts_params *tp, const or_spg_params *sp, or_ts_result *res) {
+static int ts_run_fem(const double *x, double *gamma, double *theta, int64_t T, int K, int xdim,
+                      double epssqr, const uint8_t *heldout, const or_ts_params *tp,
+                      const or_spg_params *sp, int skip_first, ts_run_out *o) {
     const int nthreads = fix_threads(sp->nthreads);
     const int64_t T2 = or_fem_reduced_T(T, tp->fem_reduce);
     if (tp->fem_type == 2 && T2 < 2) return -1;
@@ -707,7 +788,7 @@ static int or_ts_solve_fem(const double *x, double *gamma, double *theta, int64_
     double L = DBL_MAX, L_old, deltaL = DBL_MAX;
     int it, spg_sum = 0, increased = 0;
     for (it = 0; it < tp->maxit; it++) {
-        if (!tp->theta_fixed && !(it == 0 && tp->skip_first_theta))
+        if (!tp->theta_fixed && !(it == 0 && skip_first))
             or_theta_update(x, gamma, theta, T, 1, K, xdim, heldout, nthreads);
         for (int64_t t = 0; t < T; t++)
             for (int k = 0; k < K; k++) {
@@ -733,15 +814,13 @@ static int or_ts_solve_fem(const double *x, double *gamma, double *theta, int64_
         L = r.fx;
         deltaL = fabs(L - L_old);
         if (L - L_old > 0) increased++;
-        if (deltaL < tp->eps) { it++; break; }
+        if (deltaL < tp->eps) break;
     }
-    if (res) {
-        res->it = it;
-        res->spg_it_sum = spg_sum;
-        res->L = L;
-        res->deltaL = deltaL;
-        res->increased = increased;
-    }
+    o->it = it;
+    o->spg = spg_sum;
+    o->L = L;
+    o->deltaL = deltaL;
+    o->increased = increased;
     free(resid); free(b2); free(g2);
     return 0;
 }

@@ -139,15 +139,28 @@ typedef struct {
     int scalef;           /* graphh1femmodel_scalef, default 1 */
     double fem_reduce;    /* graphh1femmodel_fem_reduce (< 1: gamma solved on a coarse grid) */
     int fem_type;         /* 1 Fem, 2 FemHat (used when fem_reduce < 1; R must be 1) */
+    int annealing;        /* runs of the alternation (tssolver.h:522), default 1; every run
+                             after the first restarts from a random gamma, and the run of the
+                             smallest AIC is kept (tssolver.h:657-692) */
+    uint64_t restart_seed;/* seed of the restarts' random gamma (or_uniform); the reference
+                             re-creates its generator with seed 13 at every restart
+                             (generalvector.cpp:7-29), so every restart draws the same vector */
 } or_ts_params;
 
 typedef struct {
-    int it;               /* outer iterations performed (reference: it at break) */
-    int spg_it_sum;
-    double L;
+    int it;               /* it_last of the last run: the index of the breaking iteration, or
+                             maxit (tssolver.h:542-650: the loop breaks before it++) */
+    int spg_it_sum;       /* QPSolver::it_sum: SPG iterations of every gamma solve, all runs */
+    double L;             /* of the kept run */
     double deltaL;
-    int increased;        /* number of times L increased (reference prints an ERROR) */
+    int increased;        /* number of times L increased (reference prints an ERROR), all runs */
+    double aic;           /* GraphH1FEMModel::get_aic(L) = 2 log L + K (graphh1fem.h:443-447) */
+    int it_sum;           /* TSSolver::it_sum: it_last summed over the runs */
+    int best_run;         /* index of the kept run */
 } or_ts_result;
+
+/* n uniforms in [0, 1): SplitMix64 z, (z >> 11) * 2^-53 -- the library's generator */
+void or_uniform(uint64_t seed, int64_t n, double *out);
 
 int or_ts_solve(const double *x, double *gamma, double *theta, int64_t T, int64_t R, int K,
                 int xdim, double epssqr, const int64_t *nbr_ptr, const int32_t *nbr_idx,

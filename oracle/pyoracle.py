@@ -46,14 +46,14 @@ class TsParams(C.Structure):
     _fields_ = [
         ("maxit", C.c_int), ("eps", C.c_double), ("theta_fixed", C.c_int),
         ("skip_first_theta", C.c_int), ("scalef", C.c_int), ("fem_reduce", C.c_double),
-        ("fem_type", C.c_int),
+        ("fem_type", C.c_int), ("annealing", C.c_int), ("restart_seed", C.c_uint64),
     ]
 
 
 class TsResult(C.Structure):
     _fields_ = [
         ("it", C.c_int), ("spg_it_sum", C.c_int), ("L", C.c_double), ("deltaL", C.c_double),
-        ("increased", C.c_int),
+        ("increased", C.c_int), ("aic", C.c_double), ("it_sum", C.c_int), ("best_run", C.c_int),
     ]
 
 
@@ -96,6 +96,8 @@ def lib():
         L.or_ts_solve.restype = C.c_int
         L.or_abserr_reconstructed.argtypes = [_f64, _f64, _f64, C.c_int64, C.c_int64, C.c_int]
         L.or_abserr_reconstructed.restype = C.c_double
+        L.or_uniform.argtypes = [C.c_uint64, C.c_int64, _f64]
+        L.or_uniform.restype = None
         L.or_objective.argtypes = [_f64, _f64, C.c_int64, C.c_int64, C.c_int, C.c_double,
                                    C.c_void_p, C.c_void_p, C.c_int]
         L.or_objective.restype = C.c_double
@@ -224,7 +226,7 @@ def ts_solve(x, gamma0, theta0, T: int, R: int, K: int, xdim: int, epssqr: float
     gamma = np.array(gamma0, dtype=np.float64).ravel().copy()
     theta = np.array(theta0, dtype=np.float64).ravel().copy()
     tp = TsParams(maxit=300, eps=1e-6, theta_fixed=0, skip_first_theta=0, scalef=1,
-                  fem_reduce=1.0, fem_type=0)
+                  fem_reduce=1.0, fem_type=0, annealing=1, restart_seed=13)
     for k, v in (ts or {}).items():
         setattr(tp, k, v)
     sp = spg_params(**spg)
@@ -244,6 +246,13 @@ def abserr_reconstructed(gamma, theta, solution, T: int, R: int, K: int) -> floa
         np.ascontiguousarray(gamma, dtype=np.float64).ravel(),
         np.ascontiguousarray(theta, dtype=np.float64).ravel(),
         np.ascontiguousarray(solution, dtype=np.float64).ravel(), T, R, K))
+
+
+def uniform(seed: int, n: int):
+    """n uniforms in [0, 1) of the library's SplitMix64 generator (the restarts' random gamma)."""
+    out = np.empty(n, dtype=np.float64)
+    lib().or_uniform(seed, n, out)
+    return out
 
 
 def host_copy_bandwidth(nbytes: int = 1 << 28, reps: int = 3, nthreads: int = 1) -> float:

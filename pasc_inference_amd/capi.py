@@ -65,12 +65,14 @@ class SpgResult(C.Structure):
 
 class TsParams(C.Structure):
     _fields_ = [("maxit", C.c_int32), ("eps", C.c_double), ("theta_fixed", C.c_int32),
-                ("skip_first_theta", C.c_int32)]
+                ("skip_first_theta", C.c_int32), ("annealing", C.c_int32),
+                ("restart_seed", C.c_uint64)]
 
 
 class TsResult(C.Structure):
     _fields_ = [("it", C.c_int32), ("spg_it_sum", C.c_int32), ("L", C.c_double),
-                ("deltaL", C.c_double), ("increased", C.c_int32), ("aic", C.c_double)]
+                ("deltaL", C.c_double), ("increased", C.c_int32), ("aic", C.c_double),
+                ("it_sum", C.c_int32), ("best_run", C.c_int32)]
 
 
 class SeriesDesc(C.Structure):
@@ -155,6 +157,7 @@ _SIGS = {
                               C.c_double, C.c_double, C.POINTER(C.c_double),
                               C.POINTER(C.c_int32)], C.c_int),
     "pasc_generate_mask": ([C.c_uint64, C.c_int64, C.c_double, C.POINTER(C.c_uint8)], C.c_int),
+    "pasc_generate_uniform": ([C.c_uint64, C.c_int64, C.POINTER(C.c_double)], C.c_int),
     "pasc_generate_rect_image": ([C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                   C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.POINTER(C.c_int32)], C.c_int),

@@ -171,6 +171,8 @@ void pasc_ts_default_params(pasc_ts_params *p) {
     std::memset(p, 0, sizeof(*p));
     p->maxit = 300;
     p->eps = 1e-6;
+    p->annealing = 1;
+    p->restart_seed = 13;
 }
 
 static int check_params(const pasc_spg_params *p) {
@@ -547,12 +549,50 @@ int pasc_theta_update(pasc_problem *p) {
     return rc;
 }
 
+// One run of the alternation (tssolver.h:542-650): theta step, gamma step, L = fx of the gamma
+// solve (GraphH1FEMModel::get_L), stop at the first |L - L_old| < eps.  The reference breaks
+// before its it++, so it_last is the index of the breaking iteration, or maxit.
+struct TsRun {
+    int it = 0, spg = 0, increased = 0;
+    double L = 0.0, deltaL = 0.0;
+};
+
+static int ts_run(pasc_problem *p, const pasc_ts_params *tp, const pasc_spg_params *sp,
+                  bool skip_first, TsRun &o) {
+    double L = std::numeric_limits<double>::max(), L_old, deltaL = L;
+    int it, spg_sum = 0, increased = 0;
+    for (it = 0; it < tp->maxit; it++) {
+        if (!tp->theta_fixed && !(it == 0 && skip_first)) {
+            int rc = problem_theta_step(p);
+            if (rc) return rc;
+            p->theta_set = true;
+        }
+        if (!p->theta_set) return fail(PASC_ERR_STATE, "ts_solve: theta was never set");
+        pasc_spg_result r;
+        int rc = problem_gamma_step(p, *sp, &r);
+        if (rc) return rc;
+        spg_sum += r.it;
+        L_old = L;
+        L = r.fx;
+        deltaL = std::fabs(L - L_old);
+        if (L - L_old > 0) increased++;
+        if (deltaL < tp->eps) break;
+    }
+    o.it = it;
+    o.spg = spg_sum;
+    o.L = L;
+    o.deltaL = deltaL;
+    o.increased = increased;
+    return PASC_OK;
+}
+
 int pasc_ts_solve(pasc_problem *p, const pasc_ts_params *tp, const pasc_spg_params *sp,
                   pasc_ts_result *res) {
     clear_error();
     if (!p || !tp) return fail(PASC_ERR_ARG, "ts_solve: NULL argument");
     if (p->b_set) return fail(PASC_ERR_STATE, "ts_solve: the problem has an explicit b (theta would not enter it)");
     if (!p->has_data) return fail(PASC_ERR_STATE, "ts_solve: the problem has no data");
+    if (tp->annealing < 0) return fail(PASC_ERR_ARG, "ts_solve: annealing must be >= 0");
     int rc = check_params(sp);
     if (rc) return rc;
     rc = use_device(p->ctx);
@@ -561,34 +601,66 @@ int pasc_ts_solve(pasc_problem *p, const pasc_ts_params *tp, const pasc_spg_para
         rc = pasc_set_gamma_uniform(p);
         if (rc) return rc;
     }
-    // tssolver.h:542-650
-    double L = std::numeric_limits<double>::max(), L_old, deltaL = L;
-    int it, spg_sum = 0, increased = 0;
-    for (it = 0; it < tp->maxit; it++) {
-        if (!tp->theta_fixed && !(it == 0 && tp->skip_first_theta)) {
-            rc = problem_theta_step(p);
-            if (rc) return rc;
-            p->theta_set = true;
-        }
-        if (!p->theta_set) return fail(PASC_ERR_STATE, "ts_solve: theta was never set");
-        pasc_spg_result r;
-        rc = problem_gamma_step(p, *sp, &r);
+    // tssolver.h:491-715: `annealing` runs; each ends with AIC = 2 log L + K
+    // (graphh1fem.h:443-447), and with more than one run the gamma and theta of the smallest AIC
+    // are kept (tssolver.h:669-692).  Every run after the first starts from the random gamma of
+    // set_random (tssolver.h:687-689), which re-creates its generator with the same seed at
+    // every call (generalvector.cpp:7-29): every restart draws the same vector, here from
+    // pasc_generate_uniform(restart_seed).  The best run is the best of this solve (the
+    // reference compares against the AIC kept in TSData from earlier solves too).
+    const int runs = tp->annealing > 1 ? tp->annealing : 1;
+    const int64_t n = p->T * p->R * (int64_t)p->K;
+    const int nth = p->K * p->xdim;
+    std::vector<double> gbest, tbest, grand;
+    if (runs > 1) {
+        gbest.resize((size_t)n);
+        tbest.resize((size_t)nth);
+        grand.resize((size_t)n);
+        rc = pasc_generate_uniform(tp->restart_seed, n, grand.data());
         if (rc) return rc;
-        spg_sum += r.it;
-        L_old = L;
-        L = r.fx;
-        deltaL = std::fabs(L - L_old);
-        if (L - L_old > 0) increased++;
-        if (deltaL < tp->eps) { it++; break; }
+    }
+    double best_aic = std::numeric_limits<double>::max(), L_keep = 0.0, dL_keep = 0.0;
+    int it_last = 0, it_sum = 0, spg_sum = 0, increased = 0, best_run = 0;
+    for (int a = 0; a < runs; a++) {
+        TsRun o;
+        rc = ts_run(p, tp, sp, a == 0 && tp->skip_first_theta, o);
+        if (rc) return rc;
+        it_last = o.it;
+        it_sum += o.it;
+        spg_sum += o.spg;
+        increased += o.increased;
+        const double aic = 2.0 * std::log(o.L) + p->K;
+        if (runs <= 1 || aic < best_aic) {
+            if (runs > 1) {
+                rc = pasc_get_gamma(p, gbest.data(), PASC_LAYOUT_TMAJOR);
+                if (rc == PASC_OK) rc = pasc_get_theta(p, tbest.data());
+                if (rc) return rc;
+            }
+            best_aic = aic;
+            L_keep = o.L;
+            dL_keep = o.deltaL;
+            best_run = a;
+        }
+        if (a < runs - 1) {
+            rc = pasc_set_gamma(p, grand.data(), PASC_LAYOUT_TMAJOR);
+            if (rc) return rc;
+        }
+    }
+    if (runs > 1) {
+        rc = pasc_set_gamma(p, gbest.data(), PASC_LAYOUT_TMAJOR);
+        if (rc == PASC_OK) rc = pasc_set_theta(p, tbest.data());
+        if (rc) return rc;
     }
     p->solved = true;
     if (res) {
-        res->it = it;
+        res->it = it_last;
         res->spg_it_sum = spg_sum;
-        res->L = L;
-        res->deltaL = deltaL;
+        res->L = L_keep;
+        res->deltaL = dL_keep;
         res->increased = increased;
-        res->aic = 2.0 * std::log(L) + p->K;  // graphh1fem.h:443-447
+        res->aic = best_aic;
+        res->it_sum = it_sum;
+        res->best_run = best_run;
     }
     return PASC_OK;
 }

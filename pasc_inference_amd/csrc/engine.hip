@@ -395,8 +395,8 @@ static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<in
     if (ev) {
         hipEvent_t a;
         HIP_TRY(hipEventCreate(&a));
+        ev->push_back(a);  // owned by the caller's list from here on
         HIP_TRY(hipEventRecord(a, s));
-        ev->push_back(a);
     }
     void *g0 = ph.f64 && e.precision == PASC_FP32_BULK ? e.d_gamma64[0] : e.d_gamma[0];
     void *g1 = ph.f64 && e.precision == PASC_FP32_BULK ? e.d_gamma64[1] : e.d_gamma[1];
@@ -424,8 +424,8 @@ static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<in
     if (ev) {
         hipEvent_t b;
         HIP_TRY(hipEventCreate(&b));
-        HIP_TRY(hipEventRecord(b, s));
         ev->push_back(b);
+        HIP_TRY(hipEventRecord(b, s));
     }
     // A single 1-D problem: the pass ran the step in its last workgroup, and its tiles never
     // need compacting (the slots of a stopped instance are skipped), so nothing else is
@@ -457,6 +457,27 @@ static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<in
     return PASC_OK;
 }
 
+// Owners of the graph, its executable and the profiling events of a phase: released on every
+// exit path, error returns included.
+struct GraphGuard {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~GraphGuard() { reset(); }
+};
+
+struct EventsGuard {
+    std::vector<hipEvent_t> ev;
+    ~EventsGuard() {
+        for (auto x : ev) (void)hipEventDestroy(x);
+    }
+};
+
 static int run_phase(Engine &e, const PhaseCfg &ph, const pasc_spg_params &p, int max_passes) {
     hipStream_t s = S_(e);
     if (e.cur_phase != (ph.f64 ? 1 : 0) || tile_len(e, ph.f64) != e.cur_tile) {
@@ -486,24 +507,24 @@ static int run_phase(Engine &e, const PhaseCfg &ph, const pasc_spg_params &p, in
     const int every = std::max(1, p.check_every);
     std::vector<int32_t *> d_totals;
     for (auto &g : e.groups) d_totals.push_back(g.d_off + g.n);
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
-    std::vector<hipEvent_t> evs;
+    GraphGuard gg;
+    EventsGuard eg;
+    std::vector<hipEvent_t> &evs = eg.ev;
     int passes = 0;
     int rc = PASC_OK;
     while (passes < max_passes) {
         const int n = std::min(every, max_passes - passes);
         if (p.use_graph && n == every && !e.profile) {
-            if (!exec) {
+            if (!gg.exec) {
                 HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
                 for (int i = 0; i < n && rc == PASC_OK; i++) rc = enqueue_iteration(e, ph, grids);
-                hipError_t ec = hipStreamEndCapture(s, &graph);
+                hipError_t ec = hipStreamEndCapture(s, &gg.graph);
                 if (rc) return rc;
                 if (ec != hipSuccess) return fail(PASC_ERR_HIP, "graph capture: %s", hipGetErrorString(ec));
-                ec = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                ec = hipGraphInstantiate(&gg.exec, gg.graph, nullptr, nullptr, 0);
                 if (ec != hipSuccess) return fail(PASC_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ec));
             }
-            HIP_TRY(hipGraphLaunch(exec, s));
+            HIP_TRY(hipGraphLaunch(gg.exec, s));
         } else {
             for (int i = 0; i < n; i++) {
                 rc = enqueue_iteration(e, ph, grids, e.profile ? &evs : nullptr);
@@ -527,15 +548,9 @@ static int run_phase(Engine &e, const PhaseCfg &ph, const pasc_spg_params &p, in
                 shrink = true;
             }
         }
-        if (shrink && exec) {
-            (void)hipGraphExecDestroy(exec);
-            (void)hipGraphDestroy(graph);
-            exec = nullptr;
-            graph = nullptr;
-        }
+        if (shrink) gg.reset();
     }
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    gg.reset();
     if (e.stencil == STENCIL_TIME) {
         // the passes stored K - 1 rows: write the last one of every current iterate
         int64_t tmax = 1;
@@ -554,15 +569,24 @@ static int run_phase(Engine &e, const PhaseCfg &ph, const pasc_spg_params &p, in
     }
     if (!evs.empty()) {
         HIP_TRY(hipStreamSynchronize(s));
+        // Only the passes that ran work count: it + 1 of the longest-running instance.  The
+        // launches after the last instance stopped within a poll interval find no active tile
+        // (a few microseconds each) and would pull the average down.
+        std::vector<InstState> hs(e.n_inst);
+        HIP_TRY(hipMemcpy(hs.data(), e.d_state, sizeof(InstState) * e.n_inst,
+                          hipMemcpyDeviceToHost));
+        const int launched = (int)(evs.size() / 2);
+        int real = 1;
+        for (const auto &x : hs) real = std::max(real, x.it + 1);
+        real = std::min(real, launched);
         double ms = 0.0;
-        for (size_t i = 0; i + 1 < evs.size(); i += 2) {
+        for (int i = 0; i < real; i++) {
             float m = 0.f;
-            HIP_TRY(hipEventElapsedTime(&m, evs[i], evs[i + 1]));
+            HIP_TRY(hipEventElapsedTime(&m, evs[2 * i], evs[2 * i + 1]));
             ms += m;
         }
-        for (auto ev : evs) (void)hipEventDestroy(ev);
         e.prof_ms[ph.f64 ? 1 : 0] += ms;
-        e.prof_n[ph.f64 ? 1 : 0] += (int)(evs.size() / 2);
+        e.prof_n[ph.f64 ? 1 : 0] += real;
     }
     return PASC_OK;
 }
@@ -650,9 +674,11 @@ int engine_solve(Engine &e, const pasc_spg_params &p, pasc_spg_result *res_singl
     init_states(e, st, p, all, false);
     HIP_TRY(hipMemcpyAsync(e.d_state, st.data(), sizeof(InstState) * e.n_inst,
                            hipMemcpyHostToDevice, s));
-    hipEvent_t ev0, ev1;
-    HIP_TRY(hipEventCreate(&ev0));
-    HIP_TRY(hipEventCreate(&ev1));
+    EventsGuard eg;
+    eg.ev.assign(2, nullptr);
+    HIP_TRY(hipEventCreate(&eg.ev[0]));
+    HIP_TRY(hipEventCreate(&eg.ev[1]));
+    hipEvent_t ev0 = eg.ev[0], ev1 = eg.ev[1];
     HIP_TRY(hipEventRecord(ev0, s));
 
     const bool bulk32 = e.precision == PASC_FP32_BULK;
@@ -718,8 +744,6 @@ int engine_solve(Engine &e, const pasc_spg_params &p, pasc_spg_result *res_singl
     HIP_TRY(hipEventSynchronize(ev1));
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
     if (time_s) *time_s = ms * 1e-3;
     int worst = PASC_OK;
     for (int i = 0; i < e.n_inst; i++) {
@@ -1111,20 +1135,28 @@ __global__ __launch_bounds__(256) void pasc_copy_kernel(const float4 *__restrict
         if (base + q * 256 < n) b[base + q * 256] = v[q];
 }
 
+struct DevBuffers {
+    std::vector<void *> p;
+    ~DevBuffers() {
+        for (auto x : p)
+            if (x) (void)hipFree(x);
+    }
+};
+
 int device_copy_bandwidth(Ctx *ctx, int64_t bytes, int reps, double *gbps) {
     hipStream_t s = (hipStream_t)ctx->stream;
     const int64_t n = bytes / 16;
-    void *a = nullptr, *b = nullptr;
-    HIP_TRY(hipMalloc(&a, 16 * (size_t)n));
-    hipError_t e = hipMalloc(&b, 16 * (size_t)n);
-    if (e != hipSuccess) {
-        (void)hipFree(a);
-        return fail(PASC_ERR_HIP, "hipMalloc: %s", hipGetErrorString(e));
-    }
+    DevBuffers buf;
+    buf.p.assign(2, nullptr);
+    HIP_TRY(hipMalloc(&buf.p[0], 16 * (size_t)n));
+    HIP_TRY(hipMalloc(&buf.p[1], 16 * (size_t)n));
+    void *a = buf.p[0], *b = buf.p[1];
     HIP_TRY(hipMemsetAsync(a, 0, 16 * (size_t)n, s));
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    EventsGuard eg;
+    eg.ev.assign(2, nullptr);
+    HIP_TRY(hipEventCreate(&eg.ev[0]));
+    HIP_TRY(hipEventCreate(&eg.ev[1]));
+    hipEvent_t e0 = eg.ev[0], e1 = eg.ev[1];
     const unsigned grid = (unsigned)((n + 1023) / 1024);
     float best = 1e30f;
     for (int r = 0; r < reps; r++) {
@@ -1137,10 +1169,6 @@ int device_copy_bandwidth(Ctx *ctx, int64_t bytes, int reps, double *gbps) {
         HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
         if (ms < best) best = ms;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipFree(a);
-    (void)hipFree(b);
     *gbps = 2.0 * 16.0 * (double)n / (best * 1e-3) / 1e9;
     return PASC_OK;
 }

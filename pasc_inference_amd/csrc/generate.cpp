@@ -56,6 +56,13 @@ extern "C" int pasc_generate_markov(uint64_t seed, int64_t T, int32_t K, const d
     return PASC_OK;
 }
 
+extern "C" int pasc_generate_uniform(uint64_t seed, int64_t n, double *out) {
+    if (n < 0 || (!out && n > 0)) return pasc::fail(PASC_ERR_ARG, "generate_uniform: bad args");
+    SplitMix64 rng(seed);
+    for (int64_t i = 0; i < n; i++) out[i] = rng.uniform();
+    return PASC_OK;
+}
+
 extern "C" int pasc_generate_mask(uint64_t seed, int64_t n, double fraction, uint8_t *mask) {
     if (n < 0 || !mask) return pasc::fail(PASC_ERR_ARG, "generate_mask: bad args");
     SplitMix64 rng(seed ^ 0x6A09E667F3BCC909ull);

@@ -42,8 +42,8 @@ int main(int argc, char **argv) {
     double fem_reduce;
     opt.set_option_value("test_fem_type", &fem_type_ref, 1);
     opt.set_option_value("test_fem_reduce", &fem_reduce, 1.0);
-    if (annealing != 1) {
-        std::cout << "test_annealing: only 1 (no random restarts) is supported" << std::endl;
+    if (annealing < 1) {
+        std::cout << "test_annealing has to be >= 1" << std::endl;
         return 1;
     }
     std::vector<double> theta_list;
@@ -69,6 +69,7 @@ int main(int argc, char **argv) {
 
         model::GraphH1FEMModel<HipVector> mymodel(mydata, epssqr_list[0], opt);
         solver::TSSolver<HipVector> mysolver(mydata, mymodel, opt);
+        mysolver.set_annealing(annealing);  // TSSolver<PetscVector> mysolver(mydata, annealing), test_signal1D.cpp:237
         // gamma0: uniform (the reference draws PETSc random numbers with seed 13)
         PASC_TRY(pasc_set_gamma_uniform(mydata.get_problem()));
         if (given_theta) {

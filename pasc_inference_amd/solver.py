@@ -330,6 +330,13 @@ def generate_mask(seed: int, n: int, fraction: float):
     return m
 
 
+def generate_uniform(seed: int, n: int):
+    """n uniforms in [0, 1) of the library's SplitMix64 generator (the annealing restarts)."""
+    u = np.empty(n)
+    check(lib().pasc_generate_uniform(seed, n, dptr(u)))
+    return u
+
+
 def generate_rect_image(seed: int, width: int, height: int, K: int, n_rect: int, sigma: float):
     x = np.empty(width * height * 3)
     col = np.empty(K * 3)

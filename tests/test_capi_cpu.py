@@ -64,6 +64,17 @@ def test_markov_generator_matches_its_definition():
     np.testing.assert_array_equal(x, x2)
 
 
+def test_uniform_generator_matches_its_definition_and_the_oracle(oracle):
+    """pasc_generate_uniform (the random gamma of the annealing restarts) is (z >> 11) * 2^-53
+    of SplitMix64, the same stream the oracle's restarts draw."""
+    from pasc_inference_amd.solver import generate_uniform
+    u = generate_uniform(13, 1000)
+    g = _splitmix(13)
+    np.testing.assert_array_equal(u, [(next(g) >> 11) * 2.0 ** -53 for _ in range(1000)])
+    np.testing.assert_array_equal(u, oracle.uniform(13, 1000))
+    assert u.min() >= 0.0 and u.max() < 1.0
+
+
 def test_other_generators():
     from pasc_inference_amd.solver import (generate_batch_params, generate_mask,
                                            generate_rect_image)

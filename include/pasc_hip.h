@@ -203,6 +203,17 @@ int pasc_set_linear_term(pasc_problem *p, const double *b, int layout);
 /* ---- the theta step: updatebeforesolve_thetasolver (graphh1fem.cpp:305-396), FP64 sums ---- */
 int pasc_theta_update(pasc_problem *p);
 
+/* ---- the theta step without device allocations ----
+ * No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  The same theta as pasc_theta_update, bit for bit,
+ * but the partial sums live in a static device array of 4 MiB per device (set up by
+ * pasc_ctx_create, held until the process ends) instead of the problem's scratch buffer, which
+ * pasc_theta_update grows with hipMalloc.  Problems of T*R above
+ * 256 * (524288 / (K (1 + xdim)) - 1) steps (22.4 million at K = 3, xdim = 1) are rejected with
+ * PASC_ERR_ARG.  All calls on one device share its array: calls that overlap on one device
+ * race. */
+int pasc_theta_update_noalloc(pasc_problem *p);
+
 /* ---- outer alternation: TSSolver<PetscVector>::solve (tssolver.h:491-715) ---- */
 void pasc_ts_default_params(pasc_ts_params *p);
 int pasc_ts_solve(pasc_problem *p, const pasc_ts_params *tp, const pasc_spg_params *sp,

@@ -35,6 +35,7 @@ def _units():
     units.append((os.path.join(CSRC, "engine.hip"), "engine.o", []))
     units.append((os.path.join(CSRC, "adjoint.hip"), "adjoint.o", []))
     units.append((os.path.join(CSRC, "fem.hip"), "fem.o", []))
+    units.append((os.path.join(CSRC, "noalloc.hip"), "noalloc.o", []))
     units.append((os.path.join(CSRC, "capi.hip"), "capi.o", []))
     units.append((os.path.join(CSRC, "generate.cpp"), "generate.o", []))
     return units

@@ -123,6 +123,7 @@ _SIGS = {
                          C.c_int32], C.c_int),
     "pasc_fem_prolongate": ([vp, vp, vp, C.c_int64, C.c_int64, C.c_int32, C.c_int64, C.c_int64,
                              C.c_int32], C.c_int),
+    "pasc_theta_update_noalloc": ([vp], C.c_int),
     "pasc_ts_default_params": ([C.POINTER(TsParams)], None),
     "pasc_ts_solve": ([vp, C.POINTER(TsParams), C.POINTER(SpgParams), C.POINTER(TsResult)],
                       C.c_int),

@@ -113,6 +113,13 @@ int pasc_ctx_create(int device, pasc_ctx **out) {
     c->c.n_cu = ncu;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->c.arch = prop.gcnArchName;
+    // the device address of the static scratch of pasc_theta_update_noalloc on this device
+    const int rc = noalloc_init(device);
+    if (rc) {
+        (void)hipStreamDestroy(s);
+        delete c;
+        return rc;
+    }
     *out = c;
     return PASC_OK;
 }
@@ -545,6 +552,32 @@ int pasc_theta_update(pasc_problem *p) {
     int rc = use_device(p->ctx);
     if (rc) return rc;
     rc = problem_theta_step(p);
+    if (rc == PASC_OK) p->theta_set = true;
+    return rc;
+}
+
+int pasc_theta_update_noalloc(pasc_problem *p) {
+    clear_error();
+    if (!p) return fail(PASC_ERR_ARG, "theta_update_noalloc: NULL problem");
+    if (!p->gamma_set) return fail(PASC_ERR_STATE, "theta_update_noalloc: gamma was never set");
+    if (!p->has_data) return fail(PASC_ERR_STATE, "theta_update_noalloc: the problem has no data");
+    int rc = use_device(p->ctx);
+    if (rc) return rc;
+    double th[PASC_MAX_K * 3];
+    InstDesc &d = p->e.h_desc[0];
+    if (p->fem_type == PASC_FEM_NONE) {
+        bool f32 = false;
+        const void *g = engine_current_gamma(p->e, 0, &f32);
+        rc = rows_theta_noalloc(p->e, g, f32, d, (const double *)p->e.d_x64, th);
+    } else {
+        rc = rows_theta_noalloc(p->e, p->d_g1, false, p->fine, p->d_x1, th);
+    }
+    if (rc) return rc;
+    for (int i = 0; i < p->K * p->xdim; i++) {
+        d.theta[i] = th[i];
+        p->fine.theta[i] = th[i];
+    }
+    rc = engine_update_desc(p->e);
     if (rc == PASC_OK) p->theta_set = true;
     return rc;
 }

@@ -214,6 +214,12 @@ int rows_labels(Engine &e, const void *g, bool f32, const InstDesc &d, int32_t *
 int rows_abserr(Engine &e, const void *g, bool f32, const InstDesc &d, const double *solution,
                 double *abserr);
 
+// noalloc.hip: the theta step with its partial sums in a static device array
+int noalloc_init(int device);  // once per device, with the device current
+int64_t noalloc_theta_max_T(int K, int xdim);
+int rows_theta_noalloc(Engine &e, const void *g, bool f32, const InstDesc &d, const double *x,
+                       double *theta_out);
+
 // fem.hip
 struct FemGeom {
     int type;            // PASC_FEM_CONSTANT or PASC_FEM_HAT

@@ -7,14 +7,28 @@ and `value` is the faster of the two (`path` names it); plus the batched epsilon
 (256 uneven series x 16 epsilons) against the plain FP64 per-solve path on the same GPU.
 
 A "step" is one complete gamma solve from gamma0 to KKT <= 1e-6 with the data resident in HBM.
+Errors are stated against the FP64 CPU oracle's minimiser (KKT <= 1e-10), read from the
+committed fixture tests/golden/c2_T1e7_K3_eps100.npz.  A second timed figure solves to the KKT
+level at which gamma is within 1e-6 of that minimiser.
+
+The roofline of the dominant kernel (the fused SPG pass of the faster path) divides the bytes
+the pass moves per launch by its average launch time (hipEvents around every pass of one solve,
+on the engine's stream): HBM traffic measured in this run by two rocprofv3 counter passes
+(FETCH_SIZE, WRITE_SIZE, scaled by what the counters read for the copy kernel) when rocprofv3
+is available, else the minimum the pass must move, (2 (K-1) T + T) s.
 Prints one JSON line.
 """
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -23,6 +37,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+C2_FIXTURE = os.path.join(ROOT, "tests", "golden", "c2_T1e7_K3_eps100.npz")
+COPY_BYTES = 1 << 30
 
 
 def parse():
@@ -32,23 +48,32 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--T", type=int, default=10_000_000)
     ap.add_argument("--kkt", type=float, default=1e-6)
+    ap.add_argument("--kkt-gamma", type=float, default=2e-7,
+                    help="KKT level of the second timed figure (gamma within 1e-6 of the "
+                         "oracle's minimiser)")
     ap.add_argument("--no-scan", action="store_true")
     ap.add_argument("--no-graph-c5", action="store_true",
                     help="skip the image-graph solve reported beside the headline")
     ap.add_argument("--c5-eps", type=float, default=10.0)
     ap.add_argument("--scan-series", type=int, default=256)
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--plain-sample", type=int, default=32,
-                    help="instances of the scan timed on the plain FP64 per-solve path")
+    ap.add_argument("--plain-per-cell", type=int, default=2,
+                    help="instances of every (K, epsilon) cell of the scan timed on the plain "
+                         "FP64 per-solve path")
     ap.add_argument("--scan-maxit", type=int, default=20000,
-                    help="iteration cap of every scan solve, identical on both paths")
+                    help="iteration cap of every scan solve, identical on both paths (FP32 and "
+                         "FP64 polish iterations together on the batched path)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=24)
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 counter passes (roofline on the minimum bytes)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch iterations one by one instead of replaying hipGraphs")
     ap.add_argument("--profile-run", action="store_true",
-                    help="only a fixed number of fused passes of the T=1e7 solve in each "
-                         "precision plus a copy kernel (for rocprofv3 traces and counters)")
+                    help="a copy kernel, then --passes fused passes of the T=1e7 solve in each "
+                         "precision (for rocprofv3 traces and counters)")
+    ap.add_argument("--passes", type=int, default=200,
+                    help="--profile-run: passes per precision (0: solve to the KKT target)")
     ap.add_argument("--scan-profile", action="store_true",
                     help="run only the batched scan (for rocprofv3 traces of the batched engine)")
     ap.add_argument("--graph-profile", action="store_true",
@@ -71,14 +96,16 @@ def c2_workload(args):
     return x, lab, np.array(c["levels"]), c["eps"]
 
 
-def _timed_solves(args, P, params, label):
+def _timed_solves(args, P, params, label, steps=None, warmup=None):
     """W untimed warmup solves, then K timed solves from gamma0 = 1/K (wall clock around each,
     device synchronised on both sides); returns the times and the last result."""
-    for _ in range(args.warmup):
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
+    for _ in range(warmup):
         P.set_gamma_uniform()
         P.gamma_solve(**params)
     times, res = [], None
-    for _ in range(args.steps):
+    for _ in range(steps):
         P.set_gamma_uniform()
         P.ctx.synchronize()
         t0 = time.perf_counter()
@@ -91,7 +118,8 @@ def _timed_solves(args, P, params, label):
 
 
 def _profiled_pass(P, params, f64):
-    """One more solve with hipEvents around every fused pass: average pass time (s)."""
+    """One more solve with hipEvents around every fused pass: average pass time (s) over the
+    passes that ran work."""
     P.set_profiling(True)
     P.set_gamma_uniform()
     r = P.gamma_solve(**params)
@@ -101,35 +129,64 @@ def _profiled_pass(P, params, f64):
     return r.pass_time_fp32_s / max(r.passes_fp32, 1), r.passes_fp32
 
 
-def _errors(g, f, kkt, gref, fref, truth, T, K):
-    lab = g.reshape(T, K).argmax(1)
-    labref = gref.reshape(T, K).argmax(1)
-    return dict(gamma_maxabs=float(np.max(np.abs(g - gref))), f_rel=float(abs(f - fref) / abs(fref)),
-                label_agreement=float(np.mean(lab == labref)),
-                label_vs_truth=float(np.mean(lab == truth)), kkt_fp64=float(kkt))
+def _load_fixture(T):
+    if T != 10_000_000 or not os.path.exists(C2_FIXTURE):
+        return None
+    fx = np.load(C2_FIXTURE)
+    lab = np.repeat(fx["label_values"], np.diff(np.r_[fx["label_starts"], T])).astype(np.int32)
+    return dict(fx=fx, labels=lab)
 
 
-def run_c2(args, ctx):
+def _errors_vs_oracle(P, x, truth, T, K, fix):
+    """Errors of the problem's current gamma against the oracle's minimiser (fixture): gamma
+    max-abs at the fixture's sample, f relative (explicit FP64 evaluation), argmax agreement on
+    every step, the per-cluster sums; plus the KKT residual and the labels against the truth."""
+    g = P.get_gamma().reshape(T, K)
+    f, kkt = P.objective()
+    lab = P.labels()
+    out = dict(kkt_fp64=float(kkt), label_vs_truth=float(np.mean(lab == truth)))
+    if fix is None:
+        out["oracle"] = None
+        return out
+    fx = fix["fx"]
+    f_star = float(fx["f_star"])
+    out.update(
+        gamma_maxabs=float(np.abs(g[fx["sample_idx"]] - fx["sample_gamma"]).max()),
+        f_rel=float(abs(f - f_star) / abs(f_star)),
+        label_agreement=float(np.mean(lab == fix["labels"])),
+        sum_gamma_maxabs=float(np.abs(g.sum(0) - fx["sum_gamma"]).max()),
+        sum_x_gamma_maxabs=float(np.abs(x @ g - fx["sum_x_gamma"]).max()),
+    )
+    out["meets_bar"] = bool(out["gamma_maxabs"] <= 1e-6 and out["f_rel"] <= 1e-8 and
+                            out["label_agreement"] >= 0.9999)
+    return out
+
+
+def run_c2(args, ctx, tune_cache):
     from pasc_inference_amd import capi
     from pasc_inference_amd.solver import Problem
     x, truth, theta, eps = c2_workload(args)
     T, K = args.T, theta.size
+    fix = _load_fixture(T)
     tune = {}
     fast = dict(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt, check_every=32,
                 use_graph=0 if args.no_graph else 1, kkt_every=4)
+    bar = dict(fast, kkt_tol=args.kkt_gamma)
+    bar_steps = max(1, min(args.steps, 5))
 
     # FP32-bulk path (FP32 storage and bulk arithmetic, FP64 reductions, FP64 polish)
     P32 = Problem(ctx, x, K, precision=capi.FP32_BULK, epssqr=eps)
     P32.set_theta(theta)
     if not args.no_autotune:
         P32.set_gamma_uniform()
-        tune["fp32_bulk"] = P32.autotune(args.tune_cache)
+        tune["fp32_bulk"] = P32.autotune(tune_cache)
         log(args, f"autotune fp32-bulk: {tune['fp32_bulk']}")
     t32, r32 = _timed_solves(args, P32, fast, "fp32-bulk solve")
-    g32 = P32.get_gamma()
-    f32, kkt32 = P32.objective()
+    err32 = _errors_vs_oracle(P32, x, truth, T, K, fix)
     pass32, n32 = _profiled_pass(P32, fast, False)
     pass32_polish, _ = _profiled_pass(P32, fast, True)
+    tb32, rb32 = _timed_solves(args, P32, bar, "fp32-bulk gamma-bar solve", bar_steps, 0)
+    errb32 = _errors_vs_oracle(P32, x, truth, T, K, fix)
     P32.close()
 
     # plain FP64 per-solve path: default launch shape, host poll every iteration, no graph
@@ -140,52 +197,112 @@ def run_c2(args, ctx):
     # FP64 path: autotuned launch shape, graph-captured iteration loop
     if not args.no_autotune:
         P64.set_gamma_uniform()
-        tune["fp64"] = P64.autotune(args.tune_cache)
+        tune["fp64"] = P64.autotune(tune_cache)
         log(args, f"autotune fp64: {tune['fp64']}")
     t64g, r64g = _timed_solves(args, P64, fast, "fp64 graph solve")
-    g64 = P64.get_gamma()
-    f64, kkt64 = P64.objective()
+    err64 = _errors_vs_oracle(P64, x, truth, T, K, fix)
     pass64, n64 = _profiled_pass(P64, fast, True)
-    # FP64 reference solve (tight KKT) for the error statement
-    P64.set_gamma_uniform()
-    rref = P64.gamma_solve(maxit=50000, stop_mask=capi.STOP_KKT, kkt_tol=1e-10, check_every=32,
-                           use_graph=0 if args.no_graph else 1)
-    log(args, f"fp64 reference solve: it={rref.it} kkt={rref.kkt:.2e}")
-    gref = P64.get_gamma()
-    fref, kref = P64.objective()
+    tb64, rb64 = _timed_solves(args, P64, bar, "fp64 gamma-bar solve", bar_steps, 0)
+    errb64 = _errors_vs_oracle(P64, x, truth, T, K, fix)
     P64.close()
-    err32 = _errors(g32, f32, kkt32, gref, fref, truth, T, K)
-    err64 = _errors(g64, f64, kkt64, gref, fref, truth, T, K)
-    traffic = {}
-    tf = os.path.join(ROOT, "profiles", "traffic_fused_pass_fp32_T1e7_K3.json")
-    if os.path.exists(tf):
-        try:
-            tj = json.load(open(tf))
-            traffic = {"fp32": tj.get("bytes_per_launch"), "fp64": tj.get("fp64_bytes_per_launch")}
-        except Exception:
-            traffic = {}
     return dict(T=T, K=K, x=x, theta=theta, tune=tune, t32=t32, r32=r32, tp64=tp64, r64=r64,
                 t64g=t64g, r64g=r64g, pass32=pass32, n32=n32, pass32_polish=pass32_polish,
-                pass64=pass64, n64=n64, err32=err32, err64=err64, ref_kkt=float(kref),
-                ref_it=int(rref.it), traffic=traffic)
+                pass64=pass64, n64=n64, err32=err32, err64=err64,
+                bar={"kkt_tol": args.kkt_gamma,
+                     "fp64_graph": {"time_s": float(np.mean(tb64)), "iterations": rb64.it,
+                                    "errors_vs_oracle": errb64},
+                     "fp32_bulk": {"time_s": float(np.mean(tb32)),
+                                   "iterations": [rb32.it_fp32, rb32.it_fp64],
+                                   "errors_vs_oracle": errb32}},
+                oracle={"fixture": os.path.relpath(C2_FIXTURE, ROOT),
+                        "kkt": float(fix["fx"]["kkt"]), "it": int(fix["fx"]["it"]),
+                        "f_star": float(fix["fx"]["f_star"])} if fix else None)
+
+
+# ---- HBM traffic of the pass kernels, measured by rocprofv3 counters in this run --------------
+def _counter_rows(d):
+    """(kernel name, counter value) of every dispatch in the counter CSV files under d."""
+    rows = []
+    for fn in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(fn, newline="") as f:
+            for row in csv.DictReader(f):
+                name = row.get("Kernel_Name") or row.get("Kernel-Name") or ""
+                val = row.get("Counter_Value") or row.get("Counter-Value")
+                if val is not None:
+                    rows.append((name, float(val)))
+    return rows
+
+
+def measure_traffic(args, tune_cache):
+    """Two counter passes (FETCH_SIZE, WRITE_SIZE; separate runs, as the counters must not be
+    mixed) over `bench.py --profile-run`: a 1 GiB copy kernel and 200 fused passes per
+    precision with the shapes this run's autotuner chose.  Bytes per dispatch of each pass
+    kernel, each counter scaled by known bytes / counter bytes of the copy kernel."""
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None, "rocprofv3 not found"
+    out, tmp = {}, tempfile.mkdtemp(prefix="pasc_traffic_")
+    env = dict(os.environ, TMPDIR="/tmp")
+    try:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(tmp, counter.lower())
+            cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "c", "--",
+                   sys.executable, os.path.join(ROOT, "bench.py"), "--profile-run",
+                   "--passes", "200", "--T", str(args.T)]
+            if tune_cache:
+                cmd += ["--tune-cache", tune_cache]
+            try:
+                r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True,
+                                   timeout=180)
+            except subprocess.TimeoutExpired:
+                return None, f"rocprofv3 --pmc {counter} timed out"
+            if r.returncode != 0 and "profile-run done" not in r.stdout:
+                return None, f"rocprofv3 --pmc {counter} rc={r.returncode}: {r.stderr[-300:]}"
+            per = {}
+            for name, v in _counter_rows(d):
+                per.setdefault(name, []).append(v)
+            copy = [v for n, v in per.items() if "pasc_copy_kernel" in n]
+            if not copy:
+                return None, f"no copy-kernel dispatch in the {counter} output"
+            scale = COPY_BYTES / float(np.median(copy[0]))
+            out[counter] = {n: float(np.median(v)) * scale for n, v in per.items()
+                            if "spg_pass_1d" in n}
+            out[counter + "_scale"] = scale
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    kernels = {}
+    for n, fb in out["FETCH_SIZE"].items():
+        wb = out["WRITE_SIZE"].get(n)
+        if wb is not None:
+            kernels[n] = {"read_bytes": fb, "write_bytes": wb, "bytes": fb + wb}
+    return {"kernels": kernels, "fetch_scale": out["FETCH_SIZE_scale"],
+            "write_scale": out["WRITE_SIZE_scale"]}, None
+
+
+def _pass_kernel_name(s, K, shape):
+    pf = "true" if shape.get("prefetch") else "false"
+    return f"spg_pass_1d<{s}, {s}, {K}, {shape['chunk']}, {shape['block']}, {pf}>"
 
 
 def _roofline(kernel, s, n, T, avg, traffic, copy_gbps):
-    """The dominant kernel against the HBM roofline: achieved = algorithmic bytes per launch
-    (6*n*s, the vectors the reference iteration moves) / average launch time."""
-    alg = 6.0 * n * s
-    # K - 1 rows of gamma_{i-1} read and of gamma_i written (the last row is 1 - the others),
-    # data read
+    """The dominant kernel against the HBM roofline: bytes moved per launch (measured HBM
+    traffic when the counters ran in this run, else the minimum the pass moves) / average
+    launch time."""
     K = n // T
     own = (2.0 * (K - 1 if K >= 2 else K) * T + T) * s
-    out = {"bound": "hbm", "achieved": alg / avg / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-           "frac": alg / avg / 1e9 / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
-           "avg_launch_us": avg * 1e6, "algorithmic_bytes_per_launch": alg,
-           "own_min_bytes_per_launch": own,
-           "own_min_frac": own / avg / 1e9 / HBM_PEAK_GBS, "device_copy_gbps": copy_gbps}
+    basis = traffic if traffic else own
+    out = {"bound": "hbm", "achieved": basis / avg / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": basis / avg / 1e9 / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
+           "bytes_basis": ("HBM traffic per launch measured in this run (rocprofv3 FETCH_SIZE + "
+                           "WRITE_SIZE, scaled by the copy kernel)") if traffic else
+                          "minimum bytes the pass moves, (2 (K-1) T + T) s",
+           "avg_launch_us": avg * 1e6, "own_min_bytes_per_launch": own,
+           "own_min_frac": own / avg / 1e9 / HBM_PEAK_GBS, "device_copy_gbps": copy_gbps,
+           # the reference iteration moves 6 n s bytes (gamma and g read twice, written once);
+           # this pass keeps g and d on chip, so the rate below is reference-equivalent only
+           "reference_equivalent_gbps": 6.0 * n * s / avg / 1e9}
     if traffic:
-        out["traffic_gbps"] = traffic / avg / 1e9
-        out["traffic_frac"] = traffic / avg / 1e9 / HBM_PEAK_GBS
+        out["traffic_over_min"] = traffic / own
     return out
 
 
@@ -209,19 +326,23 @@ def run_graph_c5(args, ctx):
     pass_s, npass = _profiled_pass(P, params, True)
     P.close()
     n = w * h * K
-    alg = 6.0 * n * 8
-    # the two graph kernels move gamma_{i-1}, d_{i-1} (read), gamma_i, d_i (written) and the
-    # data (three channels): (4 n + 3 R) * 8 bytes per iteration
+    # the graph kernels move gamma_{i-1}, d_{i-1} (read), gamma_i, d_i (written) and the data
+    # (three channels): (4 n + 3 R) * 8 bytes per iteration
     own = (4.0 * n + 3.0 * w * h) * 8
     return dict(workload=f"gamma QP solve, 1024x1024 RGB image, 4-neighbour grid, K={K}, xdim=3, "
                          f"epsilon {args.c5_eps:g}, theta fixed, gamma0=1/K, KKT<={args.kkt:g}",
                 time_s=float(np.mean(times)), iterations=int(r.it), kkt=float(r.kkt),
-                label_accuracy=acc, pass_us=pass_s * 1e6,
-                algorithmic_gbps=alg / pass_s / 1e9, own_min_gbps=own / pass_s / 1e9,
+                label_accuracy=acc, pass_us=pass_s * 1e6, own_min_gbps=own / pass_s / 1e9,
                 kernels="spg_pass_grid_dir + spg_pass_grid_ad (+ spg_step) per iteration")
 
 
 def run_scan(args, ctx):
+    """The epsilon scan: every series x every epsilon, batched FP32-bulk (one engine, graph-
+    captured, FP32 and FP64-polish iterations capped together at --scan-maxit) against the
+    plain FP64 per-solve path (default launch shape, host poll every iteration, capped at the
+    same --scan-maxit) on a stratified sample of the same instances: --plain-per-cell series of
+    every (K, epsilon) cell, each cell's plain time and certified share scaled by its instance
+    count.  Rates count certified solves only (KKT <= target reached)."""
     from pasc_inference_amd import capi
     from pasc_inference_amd.solver import SCAN_EPS, Batch, Problem, generate_batch_params, \
         generate_markov
@@ -233,9 +354,11 @@ def run_scan(args, ctx):
         xs.append(x)
         ths.append(levels.copy())
     eps = SCAN_EPS
-    n_inst = args.scan_series * eps.size
-    params = dict(maxit=args.scan_maxit, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt,
-                  check_every=64, use_graph=0 if args.no_graph else 1, kkt_every=4)
+    n_eps = eps.size
+    n_inst = args.scan_series * n_eps
+    params = dict(maxit=args.scan_maxit, polish_maxit=-1, stop_mask=capi.STOP_KKT,
+                  kkt_tol=args.kkt, check_every=64, use_graph=0 if args.no_graph else 1,
+                  kkt_every=4)
     B = Batch(ctx, xs, K, ths, eps, precision=capi.FP32_BULK)
     if not args.no_autotune:
         log(args, f"autotune batch: {B.autotune(args.tune_cache)}")
@@ -244,44 +367,59 @@ def run_scan(args, ctx):
     res, tdev = B.solve(**params)
     ctx.synchronize()
     tb = time.perf_counter() - t0
-    ok = sum(1 for r in res if r.status == 0 and r.kkt <= args.kkt)
+    cert = np.array([r.status == 0 and r.kkt <= args.kkt for r in res]).reshape(-1, n_eps)
     its = np.array([r.it for r in res])
     B.close()
-    log(args, f"batched scan: {tb:.3f}s for {n_inst} solves, {ok} certified, it max {its.max()}")
+    log(args, f"batched scan: {tb:.3f}s for {n_inst} solves, {int(cert.sum())} certified, "
+              f"it max {its.max()}")
     if args.scan_profile:
-        return dict(batched_wall_s=tb, n_inst=n_inst, certified=ok, it_max=int(its.max()),
-                    it_mean=float(its.mean()))
-    # plain FP64 per-solve path on a deterministic sample of the same instances
+        return dict(batched_wall_s=tb, n_inst=n_inst, certified=int(cert.sum()),
+                    it_max=int(its.max()), it_mean=float(its.mean()))
+    # plain FP64 per-solve path on a stratified sample: --plain-per-cell series per (K, eps)
     rng = np.random.default_rng(12345)
-    pick = rng.choice(n_inst, size=min(args.plain_sample, n_inst), replace=False)
-    tp = 0.0
-    plain_ok = 0
-    for j in pick:
-        si, ei = divmod(int(j), eps.size)
-        P = Problem(ctx, xs[si], int(K[si]), precision=capi.FP64, epssqr=float(eps[ei]))
-        P.set_theta(ths[si])
-        P.set_gamma_uniform()
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        r = P.gamma_solve(maxit=args.scan_maxit, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt)
-        ctx.synchronize()
-        tp += time.perf_counter() - t0
-        plain_ok += int(r.status == 0 and r.kkt <= args.kkt)
-        P.close()
-        log(args, f"plain fp64 instance {j}: T={xs[si].size} K={K[si]} eps={eps[ei]:.3g} "
-                  f"it={r.it} kkt={r.kkt:.2e}")
-    # extrapolate the sample to all instances by each instance's share of the batch work
-    plain_rate = len(pick) / tp
-    batched_rate = n_inst / tb
-    log(args, f"plain fp64 sample: {len(pick)} solves in {tp:.3f}s")
-    return dict(batched_solves_per_s=batched_rate, plain_solves_per_s=plain_rate,
-                ratio=batched_rate / plain_rate, batched_wall_s=tb, n_inst=n_inst,
-                n_series=args.scan_series, eps_values=int(eps.size), maxit_cap=args.scan_maxit,
-                certified=ok, plain_sample=len(pick), plain_certified=plain_ok,
-                it_max=int(its.max()), it_mean=float(its.mean()),
-                note=("solves/s = instances / wall time at the same iteration cap on both paths; "
-                      "instances still above the KKT target at the cap (near-equal levels make "
-                      "the QP almost flat) count as uncertified on both"))
+    cells = {}
+    for k in sorted(set(K.tolist())):
+        ser = np.flatnonzero(K == k)
+        for e in range(n_eps):
+            pick = rng.choice(ser, size=min(args.plain_per_cell, ser.size), replace=False)
+            cells[(k, e)] = dict(n=int(ser.size), pick=[int(q) for q in pick], t=[], ok=[])
+    for (k, e), c in cells.items():
+        for si in c["pick"]:
+            P = Problem(ctx, xs[si], int(K[si]), precision=capi.FP64, epssqr=float(eps[e]))
+            P.set_theta(ths[si])
+            P.set_gamma_uniform()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            r = P.gamma_solve(maxit=args.scan_maxit, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt)
+            ctx.synchronize()
+            c["t"].append(time.perf_counter() - t0)
+            c["ok"].append(bool(r.status == 0 and r.kkt <= args.kkt))
+            P.close()
+            log(args, f"plain fp64 K={k} eps={eps[e]:.3g} series {si}: T={xs[si].size} "
+                      f"it={r.it} kkt={r.kkt:.2e} {c['t'][-1]:.3f}s")
+    # the whole scan on the plain path, estimated cell by cell
+    plain_wall = sum(c["n"] * float(np.mean(c["t"])) for c in cells.values())
+    plain_cert = sum(c["n"] * float(np.mean(c["ok"])) for c in cells.values())
+    sample_n = sum(len(c["t"]) for c in cells.values())
+    batched_rate = float(cert.sum()) / tb
+    plain_rate = plain_cert / plain_wall
+    per_eps = []
+    for e in range(n_eps):
+        pc = sum(c["n"] * float(np.mean(c["ok"])) for (k, ee), c in cells.items() if ee == e)
+        per_eps.append({"eps": float(eps[e]), "batched_certified": int(cert[:, e].sum()),
+                        "plain_certified_est": round(pc, 1), "instances": args.scan_series})
+    return dict(certified_solves_per_s={"batched": batched_rate, "plain": plain_rate},
+                ratio=batched_rate / plain_rate,
+                all_solves_per_s={"batched": n_inst / tb, "plain": n_inst / plain_wall},
+                batched_wall_s=tb, plain_wall_est_s=plain_wall, n_inst=n_inst,
+                n_series=args.scan_series, eps_values=int(n_eps), maxit_cap=args.scan_maxit,
+                certified={"batched": int(cert.sum()), "plain_est": round(plain_cert, 1)},
+                plain_sample=sample_n, plain_sample_certified=int(sum(sum(c["ok"]) for c in cells.values())),
+                it_max=int(its.max()), it_mean=float(its.mean()), per_eps=per_eps,
+                note=("rates count certified solves (KKT <= target) per second; both paths stop "
+                      "at the same iteration cap (FP32 and FP64-polish iterations together on "
+                      "the batched path); the plain path is timed on a stratified sample, "
+                      "every (K, epsilon) cell scaled by its instance count"))
 
 
 def run_cpu_baseline(args, c2):
@@ -301,8 +439,8 @@ def run_cpu_baseline(args, c2):
         out[nth] = dt / args.cpu_iters
     it_needed = c2["r64"].it
     best = min(out, key=lambda k: out[k])
-    # the same algorithmic-byte convention as the device (6 n s per iteration, s = 8) against a
-    # host copy of 2 x 512 MiB (bytes read + written per second)
+    # the reference iteration's vectors (6 n s bytes per iteration, s = 8) against a host copy
+    # of 2 x 512 MiB (bytes read + written per second)
     alg_bytes = 6.0 * T * K * 8
     stream = {str(nth): pyoracle.host_copy_bandwidth(1 << 29, 3, nth) / 1e9 for nth in sorted({1, cores})}
     return dict(
@@ -318,32 +456,39 @@ def run_cpu_baseline(args, c2):
 
 
 def profile_run(args):
-    """Fixed work for profilers: a 1 GiB copy kernel, then one solve of the bench workload per
-    path (FP32-bulk with its FP64 polish, then FP64) to the bench's KKT target, so that per-pass
-    averages cover a whole trajectory as the bench's own pass timing does.
-
-    With --tune-cache the passes use the launch shapes an earlier bench run recorded there."""
+    """Fixed work for profilers: a 1 GiB copy kernel (3 launches), then --passes fused passes
+    of the bench workload per path (FP32-bulk: FP32 phase then FP64 polish; FP64), one host
+    poll per pass.  With --passes 0 each path is solved to the bench's KKT target instead.
+    The launch shapes are those of the bench (read from --tune-cache, or tuned here)."""
     from pasc_inference_amd import capi
     from pasc_inference_amd.solver import Context, Problem
     x, _, theta, eps = c2_workload(args)
     out = {}
     with Context(0) as ctx:
-        out["copy_gbps"] = ctx.copy_bandwidth(1 << 30, 5)
+        out["copy_gbps"] = ctx.copy_bandwidth(COPY_BYTES, 3)
         for name, prec in (("fp32", capi.FP32_BULK), ("fp64", capi.FP64)):
             P = Problem(ctx, x, theta.size, precision=prec, epssqr=eps)
             P.set_theta(theta)
             P.set_gamma_uniform()
-            if args.tune_cache:
-                P.autotune(args.tune_cache)
+            # the shapes an earlier bench run recorded in the cache, or tuned here as the bench
+            # tunes them
+            P.autotune(args.tune_cache)
+            P.set_gamma_uniform()
             P.set_profiling(True)
-            r = P.gamma_solve(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt,
-                              check_every=32, kkt_every=4)
+            if args.passes > 0:
+                r = P.gamma_solve(maxit=args.passes, polish_maxit=args.passes, stop_mask=0,
+                                  check_every=1, kkt_every=4)
+            else:
+                r = P.gamma_solve(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt,
+                                  check_every=32, kkt_every=4)
             P.close()
-            t = r.pass_time_fp32_s if prec == capi.FP32_BULK else r.pass_time_fp64_s
-            n = r.passes_fp32 if prec == capi.FP32_BULK else r.passes_fp64
-            out[name + "_pass_us"] = t / max(n, 1) * 1e6
-            out[name + "_passes"] = n
+            for ph, t, n in (("fp32", r.pass_time_fp32_s, r.passes_fp32),
+                             ("fp64", r.pass_time_fp64_s, r.passes_fp64)):
+                if n:
+                    out[f"{name}_problem_{ph}_pass_us"] = t / n * 1e6
+                    out[f"{name}_problem_{ph}_passes"] = n
     print(json.dumps(out))
+    print("profile-run done", flush=True)
 
 
 def main():
@@ -366,31 +511,45 @@ def main():
               f"(requested --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
     from pasc_inference_amd.solver import Context
-    with Context(0) as ctx:
-        copy_gbps = ctx.copy_bandwidth(1 << 30, 5)
-        log(args, f"device copy kernel: {copy_gbps:.0f} GB/s")
-        c2 = run_c2(args, ctx)
-        graph = None if args.no_graph_c5 else run_graph_c5(args, ctx)
-        scan = None if args.no_scan else run_scan(args, ctx)
-    cpu = None if args.no_cpu else run_cpu_baseline(args, c2)
+    tmpdir = tempfile.mkdtemp(prefix="pasc_bench_")
+    tune_cache = args.tune_cache or os.path.join(tmpdir, "tune.jsonl")
+    try:
+        with Context(0) as ctx:
+            copy_gbps = ctx.copy_bandwidth(COPY_BYTES, 5)
+            log(args, f"device copy kernel: {copy_gbps:.0f} GB/s")
+            c2 = run_c2(args, ctx, tune_cache)
+            graph = None if args.no_graph_c5 else run_graph_c5(args, ctx)
+            scan = None if args.no_scan else run_scan(args, ctx)
+        traffic, traffic_err = (None, "skipped (--no-traffic)") if args.no_traffic else \
+            measure_traffic(args, None if args.no_autotune else tune_cache)
+        log(args, f"traffic: {traffic} {traffic_err or ''}")
+        cpu = None if args.no_cpu else run_cpu_baseline(args, c2)
+    finally:
+        shutil.rmtree(tmpdir, ignore_errors=True)
     t32, t64g = float(np.mean(c2["t32"])), float(np.mean(c2["t64g"]))
     fp64_best = t64g < t32
     best = t64g if fp64_best else t32
     T, K = c2["T"], c2["K"]
     n = T * K
     r32, r64g = c2["r32"], c2["r64g"]
-    if fp64_best:
-        solve_bytes = 6.0 * n * 8 * r64g.it
-        roof = _roofline("spg_pass_1d<double,K=3> (fused SPG iteration, FP64)", 8, n, T,
-                         c2["pass64"], c2["traffic"].get("fp64"), copy_gbps)
-        other = _roofline("spg_pass_1d<float,K=3> (fused SPG iteration, FP32 phase)", 4, n, T,
-                          c2["pass32"], c2["traffic"].get("fp32"), copy_gbps)
-    else:
-        solve_bytes = 6.0 * n * (4 * r32.it_fp32 + 8 * r32.it_fp64)
-        roof = _roofline("spg_pass_1d<float,K=3> (fused SPG iteration, FP32 phase)", 4, n, T,
-                         c2["pass32"], c2["traffic"].get("fp32"), copy_gbps)
-        other = _roofline("spg_pass_1d<double,K=3> (fused SPG iteration, FP64)", 8, n, T,
-                          c2["pass64"], c2["traffic"].get("fp64"), copy_gbps)
+    tune = c2["tune"]
+
+    def traffic_of(s, shape):
+        if not traffic or not shape:
+            return None
+        want = _pass_kernel_name(s, K, shape)
+        for name, v in traffic["kernels"].items():
+            if want in name:
+                return v["bytes"]
+        return None
+
+    sh64 = tune.get("fp64", {}).get("main")
+    sh32 = tune.get("fp32_bulk", {}).get("main")
+    roof64 = _roofline("spg_pass_1d<double,K=3> (fused SPG iteration, FP64)", 8, n, T,
+                       c2["pass64"], traffic_of("double", sh64), copy_gbps)
+    roof32 = _roofline("spg_pass_1d<float,K=3> (fused SPG iteration, FP32 phase)", 4, n, T,
+                       c2["pass32"], traffic_of("float", sh32), copy_gbps)
+    roof, other = (roof64, roof32) if fp64_best else (roof32, roof64)
     line = {
         "metric": "time-to-KKT 1e-6 at T=1e7 K=3; batched eps-scan solves/s vs plain, same GPU",
         "value": best,
@@ -413,13 +572,13 @@ def main():
         "iterations": {"fp64_graph": r64g.it, "fp32_bulk": [r32.it_fp32, r32.it_fp64],
                        "plain_fp64": c2["r64"].it},
         "speedup_vs_plain_fp64": float(np.mean(c2["tp64"])) / best,
-        "autotune": c2["tune"],
-        # whole solve against the roofline, 6*n*s algorithmic bytes per SPG iteration
-        "solve_roofline_frac": solve_bytes / best / (HBM_PEAK_GBS * 1e9),
-        "errors_vs_fp64_reference_solve": {"fp64_graph": c2["err64"], "fp32_bulk": c2["err32"],
-                                           "ref_kkt": c2["ref_kkt"], "ref_it": c2["ref_it"]},
+        "autotune": tune,
+        "errors_vs_oracle": {"fp64_graph": c2["err64"], "fp32_bulk": c2["err32"],
+                             "oracle": c2["oracle"]},
+        "time_to_gamma_1e-6": c2["bar"],
         "roofline": roof,
         "roofline_other_path": other,
+        "traffic_measurement": traffic if traffic else {"error": traffic_err},
         "fp32_polish_pass_avg_us": c2["pass32_polish"] * 1e6,
         "scan": scan,
         "graph_c5": graph,

@@ -113,7 +113,9 @@ typedef struct {
                              1e-5 * epsilon where FP32 storage stalls SPG) */
     int32_t stall_window; /* FP32-bulk: iterations without a 10% improvement of the best KKT
                              residual that end the FP32 phase (default 200) */
-    int32_t polish_maxit; /* FP32-bulk: maximum FP64 polish iterations (default maxit) */
+    int32_t polish_maxit; /* FP32-bulk: maximum FP64 polish iterations; 0 (default): maxit;
+                             -1: the FP32 phase and the polish share maxit (the polish stops
+                             once FP32 + FP64 iterations reach maxit) */
     int32_t kkt_every;    /* evaluate the KKT residual every kkt_every iterations (default 1);
                              the KKT stop then fires at the first evaluated iteration */
 } pasc_spg_params;

@@ -17,7 +17,12 @@
  *   spg          KKT certificate with exact multipliers, independent dense QP solves (scipy) on
  *                tiny problems, the f-identity, feasibility, the non-monotone bound.
  *   b / theta    closed forms; eps -> 0 gives Lloyd k-means.
- *   ts solver    monotone L on the reference data fixture samplesignal.bin.
+ *   ts solver    theta-first order (uniform gamma0 collapses theta to mean(x)); at eps = 0 the
+ *                alternation is Lloyd's k-means written in numpy (theta sequence, L, it); the
+ *                stop at the first |L - L_old| < eps with it_last = the breaking index; AIC =
+ *                2 log L + K and the annealing composition; monotone L on samplesignal.bin.
+ *   abserr       an L1 norm written out by hand, the t-major index, and the identity with the
+ *                reference's recovered-signal file.
  * The iterate sequence of the PETSc build itself is unpinned (PETSc/MPI reduction order).
  */
 #ifndef PASC_ORACLE_H

@@ -185,6 +185,7 @@ void pasc_ts_default_params(pasc_ts_params *p) {
 static int check_params(const pasc_spg_params *p) {
     if (!p) return fail(PASC_ERR_ARG, "params is NULL");
     if (p->maxit < 0) return fail(PASC_ERR_ARG, "maxit < 0");
+    if (p->polish_maxit < -1) return fail(PASC_ERR_ARG, "polish_maxit < -1");
     if (p->m < 1 || p->m > PASC_MAX_M) return fail(PASC_ERR_ARG, "m must be in [1, %d]", PASC_MAX_M);
     if (!(p->sigma2 >= p->sigma1)) return fail(PASC_ERR_ARG, "sigma2 < sigma1");
     if (!(p->alphainit > 0)) return fail(PASC_ERR_ARG, "alphainit must be > 0");

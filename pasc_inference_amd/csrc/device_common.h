@@ -39,6 +39,22 @@ template <> struct V16<double> {
     }
 };
 
+// Nontemporal 16-byte stores (global_store_dwordx4 with the nt bit).  An iterate written by
+// a pass is read again only by the next pass, after the whole vector has streamed through.
+// Measured on one MI355X: a probe stream of three read and two write vectors of 80 MiB moved
+// 5.4 TB/s with plain stores and 8.0 TB/s with nontemporal stores; in the FP64 pass they took
+// the pass from 95.8 to 87.1 us at T = 1e7, K = 3 (the FP32 pass was slower with them).
+typedef float nt_float4 __attribute__((ext_vector_type(4)));
+typedef double nt_double2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_nt(float4 *p, const float4 &v) {
+    const nt_float4 n = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(n, reinterpret_cast<nt_float4 *>(p));
+}
+__device__ __forceinline__ void store_nt(double2 *p, const double2 &v) {
+    const nt_double2 n = {v.x, v.y};
+    __builtin_nontemporal_store(n, reinterpret_cast<nt_double2 *>(p));
+}
+
 // max via the hardware min/max instructions (one VALU op instead of a compare and a select)
 __device__ inline float cmax(float a, float b) { return __builtin_fmaxf(a, b); }
 __device__ inline double cmax(double a, double b) { return __builtin_fmax(a, b); }

@@ -724,6 +724,9 @@ int engine_solve(Engine &e, const pasc_spg_params &p, pasc_spg_result *res_singl
         }
         HIP_TRY(hipGetLastError());
         init_states(e, st, p, pol, true);
+        // polish_maxit -1: the polish gets what the FP32 phase left of maxit
+        const bool shared_cap = p.polish_maxit < 0;
+        for (int i = 0; i < e.n_inst; i++) st[i].it_base = shared_cap ? it32[i] : 0;
         HIP_TRY(hipMemcpyAsync(e.d_state, st.data(), sizeof(InstState) * e.n_inst,
                                hipMemcpyHostToDevice, s));
         pasc_spg_params pp = p;
@@ -1064,8 +1067,9 @@ int engine_set_shape(Engine &e, const PassShape &s32, const PassShape &s64) {
 // workgroups on each SIMD) and for at least 4 tiles per slot (more tiles per workgroup lengthen
 // its pipeline) -- 20 passes each from the current iterate with the KKT
 // residual on every 4th pass, hipEvents around every launch.  An FP32-bulk problem runs 20 FP32
-// and 20 FP64 polish passes per candidate, and the fastest shape of each phase is kept.  The
-// iterate is restored afterwards.
+// and 20 FP64 polish passes per candidate, and the fastest shape of each phase is kept (the
+// preferred shape when it is within 3% of the fastest, see below).  The iterate is restored
+// afterwards.
 int engine_autotune(Engine &e, PassShape *s32, PassShape *s64, double us[2]) {
     if (e.stencil != STENCIL_TIME) return fail(PASC_ERR_ARG, "autotune: 1-D time graph only");
     std::vector<std::vector<double>> saved(e.n_inst);
@@ -1088,6 +1092,18 @@ int engine_autotune(Engine &e, PassShape *s32, PassShape *s64, double us[2]) {
     e.profile = true;
     double best[2] = {1e300, 1e300};
     PassShape bs[2] = {e.sh[0], e.sh[1]};
+    // The preferred shape of each phase: one-wave workgroups with the longer chunk, staged,
+    // 4096 (FP32) / 8192 (FP64) slots -- the fastest for a long series in most sessions.  It is
+    // kept whenever it is within TUNE_TIE of the fastest candidate: every shape sums the
+    // partials in its own order, so a choice that flips between near-equal shapes from run to
+    // run would change the iterates (at rounding level) and with them the iteration count.
+    constexpr double TUNE_TIE = 1.03;
+    const PassShape pref32{64, 8, 1, 4096, 1}, pref64{64, 4, 1, 8192, 1};
+    double pref_t[2] = {1e300, 1e300};
+    auto same = [](const PassShape &a, const PassShape &b) {
+        return a.block == b.block && a.chunk == b.chunk && a.prefetch == b.prefetch &&
+               a.max_slots == b.max_slots && a.min_tiles == b.min_tiles;
+    };
     for (int pi = 0; pi < 2; pi++)
     for (int si = 0; si < (pi ? NSLOT : 1); si++)
     for (int ci = 0; ci < 2; ci++)
@@ -1108,7 +1124,11 @@ int engine_autotune(Engine &e, PassShape *s32, PassShape *s64, double us[2]) {
         const double t64 = e.prof_n[1] ? e.prof_ms[1] / e.prof_n[1] : 1e300;
         if (f32 && t32 < best[0]) { best[0] = t32; bs[0] = c32; }
         if (t64 < best[1]) { best[1] = t64; bs[1] = c64; }
+        if (same(c32, pref32)) pref_t[0] = t32;
+        if (same(c64, pref64)) pref_t[1] = t64;
     }
+    if (f32 && pref_t[0] <= TUNE_TIE * best[0]) { best[0] = pref_t[0]; bs[0] = pref32; }
+    if (pref_t[1] <= TUNE_TIE * best[1]) { best[1] = pref_t[1]; bs[1] = pref64; }
     e.profile = prof;
     e.sh[0] = bs[0];
     e.sh[1] = bs[1];

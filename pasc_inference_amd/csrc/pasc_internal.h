@@ -67,6 +67,7 @@ struct InstState {
     int32_t status;
     int32_t stall;
     int32_t parity;            // state buffer holding the current iterate
+    int32_t it_base;           // iterations of an earlier phase counted against maxit
 };
 
 // Completed-slot counters of the fused step (single 1-D problems): slot j counts on sub-counter

@@ -50,7 +50,7 @@ __device__ inline void step_scalar(double sample_eps, InstState &s, const double
         else if ((P.stop_mask & PASC_STOP_ANORMGP_NORMB) && s.dAd < P.eps * s.normb) { stop = true; reason = PASC_REASON_ANORMGP_NORMB; }
         else if ((P.stop_mask & PASC_STOP_KKT) && kkt_fresh && kkt <= P.kkt_tol) { stop = true; reason = PASC_REASON_KKT; }
     }
-    if (!stop && s.it >= P.maxit) { stop = true; reason = PASC_REASON_MAXIT; }
+    if (!stop && s.it + s.it_base >= P.maxit) { stop = true; reason = PASC_REASON_MAXIT; }
     if (!stop && dd == 0.0) { stop = true; reason = PASC_REASON_STATIONARY; }
     if (!stop && P.fp32_phase) {
         // FP32 phase ends at the switch threshold or when the KKT residual stagnates

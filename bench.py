@@ -325,10 +325,11 @@ def run_graph_c5(args, ctx):
     acc = float(np.mean(g.argmax(1) == lab))
     pass_s, npass = _profiled_pass(P, params, True)
     P.close()
-    n = w * h * K
-    # the graph kernels move gamma_{i-1}, d_{i-1} (read), gamma_i, d_i (written) and the data
-    # (three channels): (4 n + 3 R) * 8 bytes per iteration
-    own = (4.0 * n + 3.0 * w * h) * 8
+    R = w * h
+    # the graph kernels store K - 1 rows of gamma and of d (the last rows follow from the
+    # simplex): gamma_{i-1}, d_{i-1} read, gamma_i, d_i written, and the data (three channels),
+    # (4 (K - 1) R + 3 R) * 8 bytes per iteration; the re-read of d_i by the d'Ad kernel is on top
+    own = (4.0 * (K - 1) * R + 3.0 * R) * 8
     return dict(workload=f"gamma QP solve, 1024x1024 RGB image, 4-neighbour grid, K={K}, xdim=3, "
                          f"epsilon {args.c5_eps:g}, theta fixed, gamma0=1/K, KKT<={args.kkt:g}",
                 time_s=float(np.mean(times)), iterations=int(r.it), kkt=float(r.kkt),

@@ -551,7 +551,7 @@ static int run_phase(Engine &e, const PhaseCfg &ph, const pasc_spg_params &p, in
         if (shrink) gg.reset();
     }
     gg.reset();
-    if (e.stencil == STENCIL_TIME) {
+    {
         // the passes stored K - 1 rows: write the last one of every current iterate
         int64_t tmax = 1;
         for (int i = 0; i < e.n_inst; i++) tmax = std::max<int64_t>(tmax, e.h_desc[i].T);

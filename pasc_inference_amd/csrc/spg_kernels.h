@@ -723,6 +723,27 @@ __device__ __forceinline__ void grid_forward_edges(const GraphGeo &G, int64_t n,
     }
 }
 
+// The graph kernels store K - 1 rows of gamma and of d as the 1-D pass stores K - 1 rows of
+// gamma: after the first pass of a phase gamma_i = gamma_{i-1} + beta d_{i-1} lies on the simplex,
+// so its last component is (1 - the others)^+, and the last component of d (the difference of
+// two points of the simplex) is minus the sum of the others.  complete_last_row writes the last
+// gamma row when a phase ends.
+template <typename C, int K>
+__device__ __forceinline__ C last_row(const C (&v)[K]) {
+    C rest = C(1);
+#pragma unroll
+    for (int k = 0; k < K - 1; k++) rest -= v[k];
+    return rest > C(0) ? rest : C(0);
+}
+
+template <typename C, int K>
+__device__ __forceinline__ C last_dir(const C (&d)[K]) {
+    C s = C(0);
+#pragma unroll
+    for (int k = 0; k < K - 1; k++) s -= d[k];
+    return s;
+}
+
 template <typename S, typename C, int K>
 __device__ __forceinline__ void grid_gamma_at(const GridArgs &A, const InstDesc &D,
                                               const InstState &st, const S *src, int64_t r,
@@ -734,8 +755,10 @@ __device__ __forceinline__ void grid_gamma_at(const GridArgs &A, const InstDesc 
     } else {
         const S *dr = reinterpret_cast<const S *>(st.parity ? A.dir1 : A.dir0) + D.goff;
         const C b = C(st.beta_apply);
+        constexpr int KS = stored_rows<K>();
 #pragma unroll
-        for (int k = 0; k < K; k++) v[k] = C(src[k * D.ld + r]) + b * C(dr[k * D.ld + r]);
+        for (int k = 0; k < KS; k++) v[k] = C(src[k * D.ld + r]) + b * C(dr[k * D.ld + r]);
+        if (KS < K) v[K - 1] = last_row(v);
     }
 }
 
@@ -749,8 +772,10 @@ __device__ __forceinline__ void grid_gamma_u(const S *src, const S *dprev, C bet
         for (int k = 0; k < K; k++) v[k] = C(src[k * ld + r]);
         project_simplex<K, C>(v);
     } else {
+        constexpr int KS = stored_rows<K>();
 #pragma unroll
-        for (int k = 0; k < K; k++) v[k] = C(src[k * ld + r]) + beta * C(dprev[k * ld + r]);
+        for (int k = 0; k < KS; k++) v[k] = C(src[k * ld + r]) + beta * C(dprev[k * ld + r]);
+        if (KS < K) v[K - 1] = last_row(v);
     }
 }
 
@@ -886,8 +911,10 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_dir(GridArgs A) {
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const C d = p[k] - g0[k];
-                dst[k * ld + r] = S(g0[k]);
-                dr[k * ld + r] = S(d);
+                if (k < stored_rows<K>()) {
+                    dst[k * ld + r] = S(g0[k]);
+                    dr[k * ld + r] = S(d);
+                }
                 ddt += (double)d * (double)d;
                 if (y[k] <= tau) off += (double)(tau - y[k]) * (double)g0[k];
                 acc[3] += 0.5 * ((double)g[k] + (double)q[k]) * (double)g0[k];
@@ -906,6 +933,14 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_dir(GridArgs A) {
             o[4] = acc[4];
         }
     }
+}
+
+// d_i at node n: K - 1 stored rows, the last rebuilt (FP64)
+template <typename S, int K>
+__device__ __forceinline__ void load_dir(const S *dr, int64_t ld, int64_t n, double (&d)[K]) {
+#pragma unroll
+    for (int k = 0; k < stored_rows<K>(); k++) d[k] = (double)dr[k * ld + n];
+    if (stored_rows<K>() < K) d[K - 1] = last_dir<double, K>(d);
 }
 
 template <typename S, typename C, int K, int NT, bool IMG>
@@ -935,26 +970,38 @@ __global__ __launch_bounds__(NT) void spg_pass_grid_ad(GridArgs A) {
                 const bool hr = col < W - 1, hd = ru + W < (uint32_t)A.geo.R;
                 const int64_t nr = hr ? r + 1 : r, nd = hd ? r + (int64_t)W : r;
                 const double w = 1.0;
+                double d0[K], d1[K], d2[K];
+                load_dir<S, K>(dr, ld, r, d0);
+                load_dir<S, K>(dr, ld, nr, d1);
+                load_dir<S, K>(dr, ld, nd, d2);
+                // the general walk's order: every component of one edge, then the next edge
+#pragma unroll
+                for (int k = 0; k < K; k++) acc[0] += d0[k] * d0[k];
 #pragma unroll
                 for (int k = 0; k < K; k++) {
-                    const double d = (double)dr[k * ld + r];
-                    const double e1 = d - (double)dr[k * ld + nr];
-                    const double e2 = d - (double)dr[k * ld + nd];
-                    acc[0] += d * d;
+                    const double e1 = d0[k] - d1[k];
                     acc[1] = hr ? acc[1] + alpha * w * e1 * e1 : acc[1];
+                }
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const double e2 = d0[k] - d2[k];
                     acc[1] = hd ? acc[1] + alpha * w * e2 * e2 : acc[1];
                 }
             } else {
+                double d0[K];
+                load_dir<S, K>(dr, ld, r, d0);
 #pragma unroll
-                for (int k = 0; k < K; k++) {
-                    const double d = (double)dr[k * ld + r];
-                    acc[0] += d * d;
-                    // d'A d = alpha * sum over the graph's edges of w (d_a - d_b)^2, each edge once
-                    grid_forward_edges<IMG>(A.geo, r, [&](int64_t n2, double w) {
-                        const double e = d - (double)dr[k * ld + n2];
+                for (int k = 0; k < K; k++) acc[0] += d0[k] * d0[k];
+                // d'A d = alpha * sum over the graph's edges of w (d_a - d_b)^2, each edge once
+                grid_forward_edges<IMG>(A.geo, r, [&](int64_t n2, double w) {
+                    double d1[K];
+                    load_dir<S, K>(dr, ld, n2, d1);
+#pragma unroll
+                    for (int k = 0; k < K; k++) {
+                        const double e = d0[k] - d1[k];
                         acc[1] += alpha * w * e * e;
-                    });
-                }
+                    }
+                });
             }
         }
       }

@@ -247,12 +247,12 @@ __global__ __launch_bounds__(256) void convert_kernel(const A *src, B *dst, int6
     if (i < n) dst[i] = B(src[i]);
 }
 
-// ---- last gamma row of the 1-D engine --------------------------------------------------------
-// The fused 1-D pass stores K - 1 rows of each iterate (the last is 1 - the sum of the others,
-// clamped at 0).  At the end of a phase this writes the full last row of every instance's
+// ---- last gamma row ----------------------------------------------------------------------------
+// The fused 1-D pass and the graph kernels store K - 1 rows of each iterate (the last is 1 - the
+// sum of the others, clamped at 0).  At the end of a phase this writes the full last row of every instance's
 // current iterate, with the pass's own arithmetic, so that the row equals the value the pass
 // used.  Instances that ran no pass in the phase (first still set) keep their rows.  grid.y
-// strides over the instances, grid.x over the steps.
+// strides over the instances, grid.x over the steps (the nodes, for a graph).
 template <typename S>
 __global__ __launch_bounds__(256) void complete_last_row(const InstDesc *desc,
                                                          const InstState *state, int n_inst,

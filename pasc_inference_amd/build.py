@@ -32,6 +32,7 @@ def _units():
             units.append((os.path.join(CSRC, "pass_inst.hip"), f"pass_{s}_{nt}.o",
                           [f"-DPASS_S={s}", f"-DPASS_NT={nt}"]))
         units.append((os.path.join(CSRC, "grid_inst.hip"), f"grid_{s}.o", [f"-DPASS_S={s}"]))
+        units.append((os.path.join(CSRC, "img_inst.hip"), f"img_{s}.o", [f"-DPASS_S={s}"]))
     units.append((os.path.join(CSRC, "engine.hip"), "engine.o", []))
     units.append((os.path.join(CSRC, "adjoint.hip"), "adjoint.o", []))
     units.append((os.path.join(CSRC, "fem.hip"), "fem.o", []))

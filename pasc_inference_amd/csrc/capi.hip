@@ -1038,7 +1038,8 @@ static void shapes_of(const Engine &e, const pasc_launch_shape &sh, PassShape &s
             s64 = to_pass(sh.polish);
         } else {
             s64 = s32;
-            s64.chunk = s32.chunk / 2;
+            // 1-D: the same bytes per thread; images: the same rows per tile
+            if (!engine_is_image(e)) s64.chunk = s32.chunk / 2;
         }
     } else {
         s32 = e.sh[0];
@@ -1047,7 +1048,6 @@ static void shapes_of(const Engine &e, const pasc_launch_shape &sh, PassShape &s
 }
 
 static int set_shape(Engine &e, const pasc_launch_shape *shape) {
-    if (e.stencil != STENCIL_TIME) return fail(PASC_ERR_ARG, "set_launch_shape: the graph path has a fixed shape");
     PassShape s32, s64;
     shapes_of(e, *shape, s32, s64);
     return engine_set_shape(e, s32, s64);
@@ -1126,7 +1126,6 @@ static int autotune_cached(Engine &e, Ctx &c, const std::string &key, int64_t si
 int pasc_set_launch_shape(pasc_problem *p, const pasc_launch_shape *shape) {
     clear_error();
     if (!p || !shape) return fail(PASC_ERR_ARG, "set_launch_shape: NULL argument");
-    if (p->graph) return fail(PASC_ERR_ARG, "set_launch_shape: the graph path has a fixed shape");
     int rc = use_device(p->ctx);
     if (rc) return rc;
     return set_shape(p->e, shape);

@@ -11,6 +11,7 @@
 #include <limits>
 #include <vector>
 
+#include "img_kernels.h"
 #include "spg_kernels.h"
 #include "step_kernels.h"
 #include "util_kernels.h"
@@ -43,6 +44,23 @@ int launch_pass_1d_double_256_(int K, const PassArgs &a, int grid, int chunk, in
 int launch_pass_1d_double_512_(int K, const PassArgs &a, int grid, int chunk, int pf, hipStream_t s);
 int launch_pass_grid_float(int K, const GridArgs &a, int grid, hipStream_t s);
 int launch_pass_grid_double(int K, const GridArgs &a, int grid, hipStream_t s);
+int launch_pass_img_float(int K, const ImgArgs &a, int grid, int pd, hipStream_t s);
+int launch_pass_img_double(int K, const ImgArgs &a, int grid, int pd, hipStream_t s);
+
+// Images (T == 1 on the implicit 4-neighbour grid, whole rows, fewer than 2^31 nodes) run the
+// fused image pass; its wave tiles are IMG_OWN columns x sh.chunk image rows, with sh.prefetch
+// raw rows in flight besides the next one.
+bool engine_is_image(const Engine &e) {
+    const GraphGeo &g = e.geo;
+    return e.stencil == STENCIL_GRID && g.T == 1 && g.width > 0 && !g.ptr &&
+           g.R % g.width == 0 && g.R < (int64_t(1) << 31);
+}
+
+static int img_ncol(const Engine &e) { return (e.geo.width + IMG_OWN - 1) / IMG_OWN; }
+static int img_nrow(const Engine &e, int rows) {
+    const int64_t h = e.geo.R / e.geo.width;
+    return (int)((h + rows - 1) / rows);
+}
 
 static int launch_pass_1d(bool f64, int K, const PassArgs &a, int grid, int block, int chunk,
                           int pf, hipStream_t s) {
@@ -65,6 +83,7 @@ static int launch_pass_1d(bool f64, int K, const PassArgs &a, int grid, int bloc
 // ------------------------------------------------------------------------------------------
 // time steps per tile of the pass kernels in a phase (shape index 1: FP64 passes)
 static int tile_len(const Engine &e, bool f64_phase) {
+    if (engine_is_image(e)) return -e.sh[f64_phase ? 1 : 0].chunk;  // image rows per tile
     if (e.stencil != STENCIL_TIME) return 256;
     const PassShape &sh = e.sh[f64_phase ? 1 : 0];
     const int ht = (3 + sh.chunk - 1) / sh.chunk;  // halo threads per side (see tile_steps)
@@ -77,7 +96,16 @@ static int engine_retile(Engine &e, bool f64_phase) {
     e.cur_tile = tl;
     e.cur_phase = f64_phase ? 1 : 0;
     int slots = 0;
+    const bool img = engine_is_image(e);
     for (auto &d : e.h_desc) {
+        if (img) {
+            // one partial slot per wave tile
+            d.msub = 1;
+            d.tile0 = slots;
+            d.ntiles = img_ncol(e) * img_nrow(e, sh.chunk);
+            slots += d.ntiles;
+            continue;
+        }
         // at most max_slots (default 2048) partial slots per instance: enough workgroups to fill
         // the device, few enough that the step reduces them quickly; at least min_tiles
         // tiles per slot
@@ -194,7 +222,8 @@ void engine_free(Engine &e) {
         if (e.d_gamma64[b]) (void)hipFree(e.d_gamma64[b]);
         if (e.d_dir[b]) (void)hipFree(e.d_dir[b]);
         if (e.d_dir64[b]) (void)hipFree(e.d_dir64[b]);
-        e.d_gamma[b] = e.d_gamma64[b] = e.d_dir[b] = e.d_dir64[b] = nullptr;
+        if (e.d_q[b]) (void)hipFree(e.d_q[b]);
+        e.d_gamma[b] = e.d_gamma64[b] = e.d_dir[b] = e.d_dir64[b] = e.d_q[b] = nullptr;
     }
     if (e.d_x64) (void)hipFree(e.d_x64);
     if (e.d_x32) (void)hipFree(e.d_x32);
@@ -255,6 +284,12 @@ int engine_set_graph(Engine &e, int64_t T, int64_t R, int32_t width, const int64
         e.geo.ptr = e.d_nbr_ptr;
         e.geo.idx = e.d_nbr_idx;
         e.geo.width = 0;
+    }
+    // images: one-wave tiles of 8 image rows in both phases (set_launch_shape and the
+    // autotuner change the rows)
+    if (engine_is_image(e)) {
+        e.sh[0] = PassShape{WAVE, 8, 2, 0, 1};
+        e.sh[1] = PassShape{WAVE, 8, 1, 0, 1};
     }
     return PASC_OK;
 }
@@ -387,11 +422,12 @@ struct PhaseCfg {
 static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<int> &grids,
                              std::vector<hipEvent_t> *ev = nullptr) {
     hipStream_t s = S_(e);
-    // single 1-D problems run the step inside the pass.  (Run in the graph path's dAd kernel,
-    // the step's serial tail cost ~8 us per iteration against ~6 us for the separate step and
-    // scan launches under graph replay, so the graph kernels keep the step kernel; single graph
-    // problems launch no scan.)
-    const bool fused = e.n_inst == 1 && e.stencil == STENCIL_TIME;
+    // single 1-D and image problems run the step inside the pass.  (Other graphs keep the step
+    // kernel: run in their dAd kernel, the step's serial tail cost ~8 us per iteration against
+    // ~6 us for the separate step and scan launches under graph replay; single graph problems
+    // launch no scan.)
+    const bool img = engine_is_image(e);
+    const bool fused = e.n_inst == 1 && (e.stencil == STENCIL_TIME || img);
     if (ev) {
         hipEvent_t a;
         HIP_TRY(hipEventCreate(&a));
@@ -412,6 +448,17 @@ static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<in
                        e.d_nactive, e.d_nactive + 1};
             const PassShape &sh = e.sh[ph.f64 ? 1 : 0];
             rc = launch_pass_1d(ph.f64, g.K, a, grids[gi], sh.block, sh.chunk, sh.prefetch, s);
+        } else if (img) {
+            void *d0 = ph.f64 && e.precision == PASC_FP32_BULK ? e.d_dir64[0] : e.d_dir[0];
+            void *d1 = ph.f64 && e.precision == PASC_FP32_BULK ? e.d_dir64[1] : e.d_dir[1];
+            const PassShape &sh = e.sh[ph.f64 ? 1 : 0];
+            const int rows = sh.chunk;
+            ImgArgs a{g0, g1, d0, d1, e.d_q[ph.f64 ? 1 : 0], e.d_desc, e.d_state, e.d_part, gv,
+                      e.geo.width, (int32_t)(e.geo.R / e.geo.width), rows, img_ncol(e),
+                      ph.kkt_every, ph.sp, fused ? e.d_done : nullptr, e.d_nactive,
+                      e.d_nactive + 1};
+            rc = ph.f64 ? launch_pass_img_double(g.K, a, grids[gi], sh.prefetch, s)
+                        : launch_pass_img_float(g.K, a, grids[gi], sh.prefetch, s);
         } else {
             void *d0 = ph.f64 && e.precision == PASC_FP32_BULK ? e.d_dir64[0] : e.d_dir[0];
             void *d1 = ph.f64 && e.precision == PASC_FP32_BULK ? e.d_dir64[1] : e.d_dir[1];
@@ -427,8 +474,8 @@ static int enqueue_iteration(Engine &e, const PhaseCfg &ph, const std::vector<in
         ev->push_back(b);
         HIP_TRY(hipEventRecord(b, s));
     }
-    // A single 1-D problem: the pass ran the step in its last workgroup, and its tiles never
-    // need compacting (the slots of a stopped instance are skipped), so nothing else is
+    // A single 1-D or image problem: the pass ran the step in its last workgroup, and its tiles
+    // never need compacting (the slots of a stopped instance are skipped), so nothing else is
     // launched.
     if (fused) {
         HIP_TRY(hipGetLastError());
@@ -478,10 +525,39 @@ struct EventsGuard {
     }
 };
 
+// The image pass reads the data term q (constant while theta is) from rows laid out like
+// gamma, formed here at the start of every phase in the phase's type.
+template <typename S>
+static int img_data_terms(Engine &e, const void *x, S *q) {
+    int64_t tmax = 1;
+    for (int i = 0; i < e.n_inst; i++) tmax = std::max<int64_t>(tmax, e.h_desc[i].T);
+    const dim3 grid((unsigned)std::min<int64_t>((tmax + 255) / 256, 4096),
+                    (unsigned)std::min(std::max(e.n_inst, 1), 65535));
+    for (auto &g : e.groups) {
+        switch (g.K) {
+#define PASC_QK(KK) \
+    case KK: hipLaunchKernelGGL((img_data_term<S, KK>), grid, dim3(256), 0, S_(e), e.d_desc, e.n_inst, x, q); break;
+            PASC_QK(1) PASC_QK(2) PASC_QK(3) PASC_QK(4) PASC_QK(5) PASC_QK(6) PASC_QK(7) PASC_QK(8)
+#undef PASC_QK
+            default: return fail(PASC_ERR_ARG, "unsupported K %d", g.K);
+        }
+    }
+    HIP_TRY(hipGetLastError());
+    return PASC_OK;
+}
+
 static int run_phase(Engine &e, const PhaseCfg &ph, const pasc_spg_params &p, int max_passes) {
     hipStream_t s = S_(e);
     if (e.cur_phase != (ph.f64 ? 1 : 0) || tile_len(e, ph.f64) != e.cur_tile) {
         int rc0 = engine_retile(e, ph.f64);
+        if (rc0) return rc0;
+    }
+    if (engine_is_image(e)) {
+        const int b = ph.f64 ? 1 : 0;
+        if (!e.d_q[b]) HIP_TRY(hipMalloc(&e.d_q[b], (ph.f64 ? 8 : 4) * (size_t)e.state_elems));
+        const void *x = ph.f64 ? (const void *)e.d_x64 : (const void *)e.d_x32;
+        const int rc0 = ph.f64 ? img_data_terms<double>(e, x, (double *)e.d_q[1])
+                               : img_data_terms<float>(e, x, (float *)e.d_q[0]);
         if (rc0) return rc0;
     }
     // initial scan (active tiles of every group)
@@ -1044,10 +1120,29 @@ static int check_shape(const PassShape &s, bool f32) {
     return PASC_OK;
 }
 
-int engine_set_shape(Engine &e, const PassShape &s32, const PassShape &s64) {
+// image pass: one-wave tiles of 1..256 image rows, 1..3 rows loaded ahead (prefetch); the
+// other fields are not used
+static int check_img_shape(const PassShape &s) {
+    if (s.block != WAVE || s.chunk < 1 || s.chunk > 256 || s.prefetch < 1 || s.prefetch > 3)
+        return fail(PASC_ERR_ARG, "image pass: block must be %d, chunk (image rows per tile) "
+                    "1..256 and prefetch 1..3, got block %d chunk %d prefetch %d", WAVE, s.block,
+                    s.chunk, s.prefetch);
+    return PASC_OK;
+}
+
+int engine_set_shape(Engine &e, const PassShape &s32_in, const PassShape &s64_in) {
     const bool f32 = e.precision == PASC_FP32_BULK;
-    int rc = f32 ? check_shape(s32, true) : PASC_OK;
-    if (rc == PASC_OK) rc = check_shape(s64, false);
+    const bool img = engine_is_image(e);
+    PassShape s32 = s32_in, s64 = s64_in;
+    if (img) {
+        // images: prefetch 0 asks for one row ahead
+        if (s32.prefetch == 0) s32.prefetch = 1;
+        if (s64.prefetch == 0) s64.prefetch = 1;
+    }
+    if (!img && e.stencil != STENCIL_TIME)
+        return fail(PASC_ERR_ARG, "set_launch_shape: graphs other than images have a fixed shape");
+    int rc = !f32 ? PASC_OK : img ? check_img_shape(s32) : check_shape(s32, true);
+    if (rc == PASC_OK) rc = img ? check_img_shape(s64) : check_shape(s64, false);
     if (rc) return rc;
     std::vector<std::vector<double>> saved(e.n_inst);
     for (int i = 0; i < e.n_inst; i++) {

@@ -130,6 +130,7 @@ struct Engine {
     int32_t *d_nbr_idx = nullptr;
     void *d_dir[2] = {nullptr, nullptr};      // grid stencil: direction ping-pong
     void *d_dir64[2] = {nullptr, nullptr};
+    void *d_q[2] = {nullptr, nullptr};        // image pass: data term rows, [0] FP32, [1] FP64
     void *d_x64 = nullptr;                    // data, FP64, rows of stride ld
     void *d_x32 = nullptr;                    // data, FP32 copy (FP32-bulk only)
     void *d_b64 = nullptr;                    // explicit linear term (single instance), FP64
@@ -199,6 +200,8 @@ int engine_adjoint(Engine &e, int inst, const uint8_t *mask, double cg_tol, int 
 // us[0] / us[1]: measured pass time of the chosen FP32 / FP64 shape (0 when not measured)
 int engine_autotune(Engine &e, PassShape *s32, PassShape *s64, double us[2]);
 int engine_set_shape(Engine &e, const PassShape &s32, const PassShape &s64);
+// the graph is an image that the fused image pass serves
+bool engine_is_image(const Engine &e);
 
 int device_copy_bandwidth(Ctx *ctx, int64_t bytes, int reps, double *gbps);
 

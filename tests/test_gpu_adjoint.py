@@ -120,11 +120,15 @@ def _torch_spg_loss(x, mask, theta, log_eps, T, K, iters):
 
 
 def test_adjoint_matches_unrolled_spg(ctx):
+    """On the first 1e4 steps of the differentiation workload's series and mask (the generators
+    are sequential streams, so the T = 1e4 draw of seed 3 is that prefix; the mask too)."""
     from pasc_inference_amd import capi
     from pasc_inference_amd.solver import Problem
     import torch
     T = 10000
-    x, mask, theta, eps = _case(T, seed=5, frac=0.1)
+    x, mask, theta, eps = _case(T)
+    xf, maskf, _, _ = _case(1000000)
+    assert np.array_equal(x, xf[:T]) and np.array_equal(mask, maskf[:T])
     K = theta.size
     P = Problem(ctx, x, K, precision=capi.FP64, heldout=mask, epssqr=eps)
     _, r = _loss(P, x, mask, theta, np.log(eps), dict(maxit=200000, stop_mask=capi.STOP_KKT,

@@ -315,7 +315,11 @@ int pasc_set_profiling(pasc_problem *p, int enable);
  * achievable HBM rate of this device, and the calibration kernel for traffic counters ---- */
 int pasc_device_copy_bandwidth(pasc_ctx *ctx, int64_t bytes, int32_t reps, double *gbps);
 
-/* ---- launch shape of the fused 1-D pass ---- */
+/* ---- launch shape of the fused 1-D pass and of the image pass ----
+ * Images (T = 1 on the implicit 4-neighbour grid) run the fused image pass: block must be 64,
+ * chunk is the number of image rows per one-wave tile (1..256; default 12) and prefetch the
+ * number of raw rows loaded ahead besides the next one (1..3, 0 = 1); max_slots and
+ * min_tiles are not used.  Other graphs have a fixed shape. */
 typedef struct {
     int32_t block;       /* threads per workgroup: 64 (one wave), 128, 256 or 512 */
     int32_t chunk;       /* time steps per thread: FP32 passes 4 or 8, FP64 passes 2 or 4 */
@@ -337,8 +341,9 @@ int pasc_set_launch_shape(pasc_problem *p, const pasc_launch_shape *shape);
 
 /* ---- autotuner ----
  * Times the fused SPG pass for the problem's (K, precision, T) over the candidate launch
- * shapes (block x chunk x prefetch x slots) and keeps the fastest for later solves; for an
- * FP32-bulk problem the FP32 phase and the FP64 polish are tuned separately.
+ * shapes (block x chunk x prefetch x slots; images: 6, 8, 12, 16, 24 rows per tile) and keeps
+ * the fastest for later solves; for an FP32-bulk problem the FP32 phase and the FP64 polish
+ * are tuned separately.
  * The choice is cached per context.  With a non-NULL cache_path, a choice recorded there for
  * the same key by an earlier process is reused without timing, and every call appends its
  * choice to the file (one JSON object per line). */

@@ -1134,7 +1134,8 @@ int pasc_set_launch_shape(pasc_problem *p, const pasc_launch_shape *shape) {
 int pasc_autotune(pasc_problem *p, const char *cache_path, pasc_launch_shape *shape) {
     clear_error();
     if (!p) return fail(PASC_ERR_ARG, "autotune: NULL problem");
-    if (p->graph) return fail(PASC_ERR_ARG, "autotune: the graph path has a fixed shape");
+    if (p->graph && !engine_is_image(p->e))
+        return fail(PASC_ERR_ARG, "autotune: graphs other than images have a fixed shape");
     if (!p->theta_set && !p->b_set) return fail(PASC_ERR_STATE, "autotune: neither theta nor b was set");
     int rc = use_device(p->ctx);
     if (rc) return rc;
@@ -1147,10 +1148,12 @@ int pasc_autotune(pasc_problem *p, const char *cache_path, pasc_launch_shape *sh
         if (rc) return rc;
     }
     const int64_t n = p->e.h_desc[0].T;
-    // keyed by device, K, precision and log2(T)
+    // keyed by device, K, precision and log2(T) (images: also the width)
     char key[160];
-    snprintf(key, sizeof(key), "%s/K%d/p%d/T2^%d", p->ctx->c.arch.c_str(), p->K, p->precision,
-             (int)std::floor(std::log2((double)n)));
+    char kind[32] = "";
+    if (p->graph) snprintf(kind, sizeof(kind), "img%d/", p->e.geo.width);
+    snprintf(key, sizeof(key), "%s/%sK%d/p%d/T2^%d", p->ctx->c.arch.c_str(), kind, p->K,
+             p->precision, (int)std::floor(std::log2((double)n)));
     return autotune_cached(p->e, p->ctx->c, key, n, cache_path, shape);
 }
 
@@ -1165,14 +1168,17 @@ int pasc_batch_set_launch_shape(pasc_batch *b, const pasc_launch_shape *shape) {
 int pasc_batch_autotune(pasc_batch *b, const char *cache_path, pasc_launch_shape *shape) {
     clear_error();
     if (!b) return fail(PASC_ERR_ARG, "batch_autotune: NULL batch");
-    if (b->e.stencil != STENCIL_TIME) return fail(PASC_ERR_ARG, "batch_autotune: the graph path has a fixed shape");
+    if (b->e.stencil != STENCIL_TIME && !engine_is_image(b->e))
+        return fail(PASC_ERR_ARG, "batch_autotune: graphs other than images have a fixed shape");
     int rc = use_device(b->ctx);
     if (rc) return rc;
     int64_t elems = 0;
     for (const auto &d : b->e.h_desc) elems += d.T * d.K;
     // keyed by device, precision, instance count and log2 of the total elements
     char key[160];
-    snprintf(key, sizeof(key), "%s/batch%d/p%d/E2^%d", b->ctx->c.arch.c_str(), b->e.n_inst,
+    char kind[32] = "";
+    if (b->e.stencil != STENCIL_TIME) snprintf(kind, sizeof(kind), "img%d/", b->e.geo.width);
+    snprintf(key, sizeof(key), "%s/%sbatch%d/p%d/E2^%d", b->ctx->c.arch.c_str(), kind, b->e.n_inst,
              b->e.precision, (int)std::floor(std::log2((double)std::max<int64_t>(elems, 1))));
     rc = engine_fill_uniform(b->e);  // the scan starts from gamma0 = 1/K
     if (rc) return rc;

@@ -216,3 +216,25 @@ def test_image_pass_rejects_bad_shapes(ctx):
         with pytest.raises(capi.PascError):
             P.set_launch_shape(block, chunk)
     P.close()
+
+
+def test_image_autotune_keeps_iterate(ctx):
+    """The autotuner times the image pass over its tile rows, keeps the iterate, and the tuned
+    problem still solves to the target."""
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Problem, generate_rect_image
+    w, h, K = 150, 90, 3
+    x, colr, _ = generate_rect_image(31, w, h, K, 7, 0.2)
+    for prec in (capi.FP64, capi.FP32_BULK):
+        P = Problem(ctx, x, K, T=1, xdim=3, grid_width=w, precision=prec, epssqr=2.0)
+        P.set_theta(colr)
+        P.set_gamma_uniform()
+        g0 = P.get_gamma()
+        sh = P.autotune()
+        print(f"image autotune prec {prec}: {sh}", flush=True)
+        for ph in ("main", "polish"):
+            assert sh[ph]["block"] == 64 and sh[ph]["chunk"] in (6, 8, 12, 16, 24)
+        assert np.array_equal(P.get_gamma(), g0)
+        r = P.gamma_solve(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=1e-10)
+        assert r.stop_reason == capi.REASON_KKT
+        P.close()

@@ -272,12 +272,22 @@ __device__ __forceinline__ P *uni(P *v) {
     return reinterpret_cast<P *>(uni((int64_t)reinterpret_cast<uintptr_t>(v)));
 }
 
-// Hand-over of a few values between workgroups within one launch, without an agent-scope fence
-// (on gfx950 a release fence writes back every dirty line of the XCD's L2).  The values are
-// agent-scope atomic stores, which write through to the device coherence point; publish_wait()
-// waits until they completed, after which the writer counts them with an atomic RMW; the
-// reader, once it observed the count, reads them with agent-scope atomic loads, which read from
-// the coherence point rather than a possibly stale line of its own XCD's L2.
+// Hand-over of a few values between workgroups within one launch (the partial sums of the
+// fused step), without an agent-scope fence (on gfx950 a release fence writes back every dirty
+// line of the XCD's L2, here the gamma rows in flight).  Each value location carries its own
+// state: it holds PART_EMPTY (a NaN whose payload no arithmetic produces) until its writer
+// publishes the value with an agent-scope atomic store; the reader, which learned from a
+// counter that the writer is done, reads the location with agent-scope atomic loads until it
+// no longer holds PART_EMPTY (settle_published), and puts PART_EMPTY back for the next launch
+// (reset_published).
+// Correctness relies only on the coherence of each single location, not on an order between
+// two locations (the value and the counter).  The writer still waits for its value stores to
+// complete before it counts (publish_wait), so that the reader's first load finds the value
+// and the wait loop is not entered in practice.  The wait is bounded: a value that does not
+// appear reads as a quiet NaN, which the step reports as PASC_ERR_NUMERIC.
+constexpr uint64_t PART_EMPTY = 0x7ff4504153430000ull;  // NaN, payload "PASC"
+constexpr int PUBLISH_SPIN_LIMIT = 1 << 16;
+
 __device__ __forceinline__ void store_coherent(double *p, double v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -288,6 +298,22 @@ __device__ __forceinline__ void publish_wait() {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);  // keeps the compiler from moving the stores
     __builtin_amdgcn_s_waitcnt(0);            // all vector memory operations completed
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+// v: a load_coherent of p issued earlier (the loads of a slot go out together); returns the
+// published value
+__device__ __forceinline__ double settle_published(const double *p, double v) {
+    for (int n = 0; __builtin_bit_cast(uint64_t, v) == PART_EMPTY; n++) {
+        if (n >= PUBLISH_SPIN_LIMIT) return __builtin_nan("");
+        __builtin_amdgcn_s_sleep(1);
+        v = load_coherent(p);
+    }
+    return v;
+}
+// marks a location empty again once its value was taken; a plain store, issued after the
+// reader's own publication so that no wait covers it (the launch boundary makes it visible to
+// the next launch's writer and reader)
+__device__ __forceinline__ void reset_published(double *p) {
+    *p = __builtin_bit_cast(double, PART_EMPTY);
 }
 
 }  // namespace pasc

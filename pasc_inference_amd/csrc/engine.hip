@@ -122,6 +122,11 @@ static int engine_retile(Engine &e, bool f64_phase) {
         e.d_part = nullptr;
         e.total_slots = slots;
         HIP_TRY(hipMalloc(&e.d_part, sizeof(double) * (size_t)std::max(slots, 1) * NPART));
+        // every partial starts unpublished (the fused step's hand-over, take_published)
+        const int64_t nw = (int64_t)std::max(slots, 1) * NPART;
+        hipLaunchKernelGGL(fill_u64, dim3((unsigned)std::min<int64_t>((nw + 255) / 256, 1024)),
+                           dim3(256), 0, S_(e), (uint64_t *)e.d_part, nw, PART_EMPTY);
+        HIP_TRY(hipGetLastError());
     }
     for (auto &g : e.groups) {
         std::vector<int32_t> ids;
@@ -193,6 +198,10 @@ int engine_alloc(Engine &e, Ctx *ctx, int precision, int stencil, int32_t grid_w
     HIP_TRY(hipMalloc(&e.d_nactive, 2 * sizeof(int32_t)));  // [0] active count, [1] dirty flag
     HIP_TRY(hipMalloc(&e.d_done, DONE_BYTES));
     HIP_TRY(hipMemsetAsync(e.d_done, 0, DONE_BYTES, S_(e)));
+    // the group partials after the counters start unpublished
+    hipLaunchKernelGGL(fill_u64, dim3(1), dim3(256), 0, S_(e),
+                       (uint64_t *)(e.d_done + DONE_WORDS), (int64_t)DONE_SUB * NPART, PART_EMPTY);
+    HIP_TRY(hipGetLastError());
     // pinned poll area: [0] active instances, [1 + g] active tiles of K group g
     HIP_TRY(hipHostMalloc((void **)&e.h_nactive, sizeof(int32_t) * (1 + PASC_MAX_K),
                           hipHostMallocDefault));

@@ -367,7 +367,7 @@ __global__ __launch_bounds__(WAVE) void spg_pass_img(ImgArgs A) {
         uint64_t gmask = (uint64_t)uni((int64_t)groups_done);
         if (gmask) {
             const int ns = uni(A.desc[0].ntiles);
-            const double *part0 = A.part + (int64_t)uni(A.desc[0].tile0) * NPART;
+            double *part0 = A.part + (int64_t)uni(A.desc[0].tile0) * NPART;
             while (gmask) {
                 const int g = __builtin_ctzll(gmask);
                 gmask &= gmask - 1;

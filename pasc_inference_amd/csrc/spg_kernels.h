@@ -502,10 +502,16 @@ __device__ __forceinline__ void fused_step(const InstDesc *desc, InstState *stat
                                            int inst, InstState *lds_state, double *lds_red) {
     const InstDesc &D = desc[inst];
     const int ns = uni(D.ntiles);
-    const double *gp = group_partials(done);
+    double *gp = group_partials(done);
     step_instance<NT>(state + inst, ns < DONE_SUB ? ns : DONE_SUB, D.alpha * D.s_kkt, sp, dirty,
                       lds_state, lds_red,
-                      [&](int i, int q) { return load_coherent(gp + (int64_t)i * NPART + q); });
+                      [&](int i, int q) { return load_coherent(gp + (int64_t)i * NPART + q); },
+                      [&](int i, int q, double v) {
+                          return settle_published(gp + (int64_t)i * NPART + q, v);
+                      });
+    // the group partials are taken: empty again for the next pass
+    const int ng = ns < DONE_SUB ? ns : DONE_SUB;
+    for (int i = threadIdx.x; i < ng * NPART; i += NT) reset_published(gp + i);
     if (threadIdx.x == 0 && !lds_state->active) atomicSub(nactive, 1);
 }
 
@@ -527,7 +533,7 @@ __device__ __forceinline__ bool count_slot(int32_t *done, int g, int ns) {
 // Each slot is reduced once, by the workgroup completing its group, so the pass's last
 // workgroup only reduces its own group and the DONE_SUB group partials.
 template <int NT>
-__device__ __forceinline__ bool reduce_group(int32_t *done, const double *part, int g, int ns,
+__device__ __forceinline__ bool reduce_group(int32_t *done, double *part, int g, int ns,
                                              double *lds_red) {
     double v[NPART] = {0.0, 0.0, 0.0, 0.0, -1.0};  // residual -1: not evaluated
     for (int j = g + (int)threadIdx.x * DONE_SUB; j < ns; j += NT * DONE_SUB) {
@@ -535,21 +541,29 @@ __device__ __forceinline__ bool reduce_group(int32_t *done, const double *part, 
 #pragma unroll
         for (int q = 0; q < NPART; q++) w[q] = load_coherent(part + (int64_t)j * NPART + q);
 #pragma unroll
+        for (int q = 0; q < NPART; q++) w[q] = settle_published(part + (int64_t)j * NPART + q, w[q]);
+#pragma unroll
         for (int q = 0; q < NPART - 1; q++) v[q] += w[q];
         const double m = w[NPART - 1];
         v[NPART - 1] = (m > v[NPART - 1] || m != m) ? m : v[NPART - 1];
     }
     block_reduce_part<NT>(v, lds_red);
-    if (threadIdx.x != 0) return false;
-    double *o = group_partials(done) + g * NPART;
+    bool last = false;
+    if (threadIdx.x == 0) {
+        double *o = group_partials(done) + g * NPART;
 #pragma unroll
-    for (int q = 0; q < NPART; q++) store_coherent(o + q, v[q]);
-    publish_wait();
-    int32_t *top = done + DONE_SUB * DONE_STRIDE;
-    const int n_top = ns < DONE_SUB ? ns : DONE_SUB;
-    if (atomicAdd(top, 1) != n_top - 1) return false;
-    __hip_atomic_store(top, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return true;
+        for (int q = 0; q < NPART; q++) store_coherent(o + q, v[q]);
+        publish_wait();
+        int32_t *top = done + DONE_SUB * DONE_STRIDE;
+        const int n_top = ns < DONE_SUB ? ns : DONE_SUB;
+        last = atomicAdd(top, 1) == n_top - 1;
+        if (last) __hip_atomic_store(top, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // the group's slots are taken: empty again for the next pass
+    for (int j = g + (int)threadIdx.x * DONE_SUB; j < ns; j += NT * DONE_SUB)
+#pragma unroll
+        for (int q = 0; q < NPART; q++) reset_published(part + (int64_t)j * NPART + q);
+    return last;
 }
 
 // PF: software pipelining across the msub tiles of a slot.  While a tile is computed, the
@@ -624,9 +638,8 @@ __global__ __launch_bounds__(NT) void spg_pass_1d(PassArgs A) {
             double *o = A.part + (int64_t)slot_out * NPART;
             if (A.done) {
                 // Single problems: publish the slot, then count it on its group's counter (the
-                // group is reduced after the tile loop).  The slot is written through to the
-                // device coherence point and counted once those stores completed; an
-                // agent-scope release fence would also write back every dirty line of this
+                // group is reduced after the tile loop; store_coherent / settle_published).  An
+                // agent-scope release fence would instead write back every dirty line of this
                 // XCD's L2 (the gamma rows in flight), which costs a sixth of the pass.
                 const int g = slot % DONE_SUB;
 #pragma unroll
@@ -648,7 +661,7 @@ __global__ __launch_bounds__(NT) void spg_pass_1d(PassArgs A) {
         uint64_t gmask = (uint64_t)uni((int64_t)groups_done);
         if (gmask) {
             const int ns = uni(A.desc[0].ntiles);
-            const double *part0 = A.part + (int64_t)uni(A.desc[0].tile0) * NPART;
+            double *part0 = A.part + (int64_t)uni(A.desc[0].tile0) * NPART;
             while (gmask) {
                 const int g = __builtin_ctzll(gmask);
                 gmask &= gmask - 1;

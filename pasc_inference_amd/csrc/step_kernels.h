@@ -21,7 +21,8 @@ __global__ __launch_bounds__(256) void spg_step(const InstDesc *desc, InstState 
         const InstDesc &D = desc[w];
         const double *pw = part + (int64_t)uni(D.tile0) * NPART;
         step_instance<256>(state + w, uni(D.ntiles), D.alpha * D.s_kkt, P, dirty, &lds_state, lds,
-                           [&](int i, int q) { return pw[(int64_t)i * NPART + q]; });
+                           [&](int i, int q) { return pw[(int64_t)i * NPART + q]; },
+                           [](int, int, double v) { return v; });
         if (nactive && threadIdx.x == 0 && !lds_state.active) atomicSub(nactive, 1);
         __syncthreads();  // lds_state is reused by the next instance
     }

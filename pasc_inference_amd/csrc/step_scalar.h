@@ -106,15 +106,17 @@ __device__ inline void step_scalar(double sample_eps, InstState &s, const double
 // One instance's step by a workgroup of NT threads.  The state is staged in LDS word by word by
 // many lanes at once, so that step_scalar's chain of dependent reads and writes costs LDS
 // latency rather than a device-memory round trip each; the partial slots are read meanwhile.
-// load_slot(i, q) returns component q of the instance's partial slot i (i < ntiles).
+// load_slot(i, q) loads component q of the instance's partial slot i (i < ntiles); settle(i,
+// q, v) turns that load into the value (the fused step waits there for a value not yet
+// published and marks the slot empty again; the step kernel returns v).
 static_assert(sizeof(InstState) % 8 == 0, "InstState is staged in 8-byte words");
 constexpr int STATE_WORDS = (int)(sizeof(InstState) / 8);
 
-template <int NT, typename LoadSlot>
+template <int NT, typename LoadSlot, typename Settle>
 __device__ __forceinline__ void step_instance(InstState *state, int ntiles, double sample_eps,
                                               const StepParams &P, int32_t *dirty,
                                               InstState *lds_state, double *lds_red,
-                                              LoadSlot load_slot) {
+                                              LoadSlot load_slot, Settle settle) {
     constexpr int WPT = (STATE_WORDS + NT - 1) / NT;  // state words per thread
     uint64_t *gw = reinterpret_cast<uint64_t *>(state);
     uint64_t *lw = reinterpret_cast<uint64_t *>(lds_state);
@@ -131,16 +133,18 @@ __device__ __forceinline__ void step_instance(InstState *state, int ntiles, doub
         double w[B][NPART];
 #pragma unroll
         for (int b = 0; b < B; b++) {
-            // past the end: read slot 0 (in bounds) and replace it by the neutral element
-            const int i = i0 + b * NT < ntiles ? i0 + b * NT : 0;
+            // past the end: the neutral element (each slot is read exactly once)
+            const int i = i0 + b * NT;
 #pragma unroll
-            for (int q = 0; q < NPART; q++) w[b][q] = load_slot(i, q);
+            for (int q = 0; q < NPART; q++)
+                w[b][q] = i < ntiles ? load_slot(i, q) : (q == NPART - 1 ? -1.0 : 0.0);
         }
 #pragma unroll
         for (int b = 0; b < B; b++) {
-            if (i0 + b * NT >= ntiles) {
+            const int i = i0 + b * NT;
+            if (i < ntiles) {
 #pragma unroll
-                for (int q = 0; q < NPART; q++) w[b][q] = q == NPART - 1 ? -1.0 : 0.0;
+                for (int q = 0; q < NPART; q++) w[b][q] = settle(i, q, w[b][q]);
             }
         }
 #pragma unroll

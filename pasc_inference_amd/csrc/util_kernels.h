@@ -247,6 +247,12 @@ __global__ __launch_bounds__(256) void convert_kernel(const A *src, B *dst, int6
     if (i < n) dst[i] = B(src[i]);
 }
 
+// ---- fill of 64-bit words (the PART_EMPTY marks of the fused step's partial sums) ----------
+static __global__ __launch_bounds__(256) void fill_u64(uint64_t *p, int64_t n, uint64_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        p[i] = v;
+}
+
 // ---- last gamma row ----------------------------------------------------------------------------
 // The fused 1-D pass and the graph kernels store K - 1 rows of each iterate (the last is 1 - the
 // sum of the others, clamped at 0).  At the end of a phase this writes the full last row of every instance's

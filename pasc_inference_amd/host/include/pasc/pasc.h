@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.h"
+#include "decomposition.h"
 #include "model.h"
 #include "solvers.h"
 #include "tsdata.h"

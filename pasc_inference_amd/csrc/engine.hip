@@ -47,13 +47,18 @@ int launch_pass_grid_double(int K, const GridArgs &a, int grid, hipStream_t s);
 int launch_pass_img_float(int K, const ImgArgs &a, int grid, int pd, hipStream_t s);
 int launch_pass_img_double(int K, const ImgArgs &a, int grid, int pd, hipStream_t s);
 
-// Images (T == 1 on the implicit 4-neighbour grid, whole rows, fewer than 2^31 nodes) run the
+// Images (T == 1 on the implicit 4-neighbour grid, whole rows, fewer than 2^31 nodes, every
+// instance's K rows within 4 GiB: the pass addresses them with 32-bit byte offsets) run the
 // fused image pass; its wave tiles are IMG_OWN columns x sh.chunk image rows, with sh.prefetch
-// raw rows in flight besides the next one.
+// raw rows in flight besides the next one.  Larger images run the node-wise graph kernels.
 bool engine_is_image(const Engine &e) {
     const GraphGeo &g = e.geo;
-    return e.stencil == STENCIL_GRID && g.T == 1 && g.width > 0 && !g.ptr &&
-           g.R % g.width == 0 && g.R < (int64_t(1) << 31);
+    if (!(e.stencil == STENCIL_GRID && g.T == 1 && g.width > 0 && !g.ptr && g.R % g.width == 0 &&
+          g.R < (int64_t(1) << 31)))
+        return false;
+    for (const auto &d : e.h_desc)
+        if ((int64_t)d.K * d.ld * 8 >= (int64_t(1) << 32)) return false;
+    return true;
 }
 
 static int img_ncol(const Engine &e) { return (e.geo.width + IMG_OWN - 1) / IMG_OWN; }

@@ -2,7 +2,7 @@
 // (blockgraphsparse.cpp:48-109 with T = 1, the graph test_image.cpp:156-168 builds), one launch
 // per iteration with the step in the last workgroup, as in the 1-D pass.
 //
-// A wave owns a tile of IMG_OWN = 61 image columns x `rows` image rows and walks down it one
+// A wave owns a tile of IMG_OWN = 61 image columns x `rows` image rows and walks along it one
 // image row at a time.  Lane l holds column c0 - 1 + l: lanes 1..61 are the owned columns, lane
 // 0 and lane 63 only rebuild gamma_i (the left and right neighbours of the owned edge columns),
 // lane 62 also forms d_i (the right end of the last owned column's edge).  Left and right
@@ -20,7 +20,9 @@
 //     end), one row past the tile (the first row of the tile below) being formed for the last
 //     vertical edges.
 // The data term q (K rows, constant while theta is) is formed once per phase by img_data_term.
-// The raw rows of gamma_{i-1}, d_{i-1} and q are loaded PD + 1 rows ahead of their use.
+// The raw rows of gamma_{i-1}, d_{i-1} and q are loaded PD rows before gamma_i is formed on
+// them.  Plain stores: nontemporal ones measured slower, although no lane reads gamma_i or d_i
+// again in the pass.
 // Every node's gamma_i, Laplacian (walk order up, left, right, down, a missing neighbour's term
 // multiplied by 0) and d_i are those of the node-wise graph kernels; the sums over the nodes
 // run in the tile order, so they differ from those kernels' in rounding only.
@@ -132,16 +134,6 @@ __device__ __forceinline__ T &at32(T *base, uint32_t i) {
                                   (uint32_t)(i * (uint32_t)sizeof(T)));
 }
 
-// gamma_i and d_i are not read again in the pass that writes them
-constexpr bool IMG_NT_STORES = false;
-template <typename S>
-__device__ __forceinline__ void img_store(S *p, S v) {
-    if constexpr (IMG_NT_STORES)
-        __builtin_nontemporal_store(v, p);
-    else
-        *p = v;
-}
-
 template <typename S, typename C, int K, bool FIRST>
 __device__ __forceinline__ void img_load(const ImgCtx<S, C, K> &c, uint32_t n,
                                          ImgRaw<S, K, FIRST> &r) {
@@ -195,7 +187,7 @@ __device__ __forceinline__ void img_tile(const ImgCtx<S, C, K> &c, int slot, int
     const int r1 = r0 + rows < H ? r0 + rows : H;  // owned rows [r0, r1)
     const int ylast = r1 < H ? r1 : H - 1;          // rows [r0, ylast] get d_i (r1: halo)
     const int ys = DOWN ? r0 : ylast;  // first row of the walk
-    const int yb = ys - STEP;                                   // the row behind the first
+    const int yb = ys - STEP;          // the row behind the first
     const uint32_t ld = c.ld;
     auto node = [&](int y) {
         const int yc = y < 0 ? 0 : (y < H ? y : H - 1);
@@ -259,8 +251,8 @@ __device__ __forceinline__ void img_tile(const ImgCtx<S, C, K> &c, int slot, int
                 const uint32_t n = (uint32_t)(y * W + col);
 #pragma unroll
                 for (int k = 0; k < KS; k++) {
-                    img_store(&at32(c.dst, k * ld + n), S(gC[k]));
-                    img_store(&at32(c.dout, k * ld + n), S(d[k]));
+                    at32(c.dst, k * ld + n) = S(gC[k]);
+                    at32(c.dout, k * ld + n) = S(d[k]);
                 }
             }
             const double wn = own ? 1.0 : 0.0;

@@ -206,7 +206,9 @@ class Problem:
     def set_launch_shape(self, block: int, chunk: int, prefetch: int = 0, max_slots: int = 0,
                          min_tiles: int = 0, polish=None):
         """main shape (FP64 problems: every pass; FP32-bulk: the FP32 phase); polish: a dict
-        for the FP64 polish of FP32-bulk problems (default: same shape, chunk / 2)."""
+        for the FP64 polish of FP32-bulk problems (default: same shape, chunk / 2).  Images
+        (grid_width, T = 1): block 64, chunk = image rows per wave tile, prefetch = rows loaded
+        ahead (1..3); the polish defaults to the same rows."""
         sh = capi.LaunchShape()
         sh.main = capi.PassShape(block, chunk, prefetch, max_slots, min_tiles, 0.0)
         if polish:

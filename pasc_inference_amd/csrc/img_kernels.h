@@ -346,7 +346,6 @@ __global__ __launch_bounds__(WAVE) void spg_pass_img(ImgArgs A) {
                 const int g = slot % DONE_SUB;
 #pragma unroll
                 for (int i = 0; i < NPART; i++) store_coherent(o + i, acc[i]);
-                publish_wait();
                 if (count_slot(A.done, g, uni(D.ntiles))) groups_done |= 1ull << g;
             } else {
 #pragma unroll

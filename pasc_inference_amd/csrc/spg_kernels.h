@@ -516,7 +516,7 @@ __device__ __forceinline__ void fused_step(const InstDesc *desc, InstState *stat
 }
 
 // Counts one completed partial slot of a single problem (its values already written with
-// store_coherent and publish_wait) on the sub-counter of its group g = slot % DONE_SUB.  True
+// store_coherent) on the sub-counter of its group g = slot % DONE_SUB.  True
 // in the workgroup that completed the group; the counter is back to 0 afterwards.  Thread 0.
 __device__ __forceinline__ bool count_slot(int32_t *done, int g, int ns) {
     const int n_g = ns / DONE_SUB + (g < ns % DONE_SUB ? 1 : 0);
@@ -553,7 +553,6 @@ __device__ __forceinline__ bool reduce_group(int32_t *done, double *part, int g,
         double *o = group_partials(done) + g * NPART;
 #pragma unroll
         for (int q = 0; q < NPART; q++) store_coherent(o + q, v[q]);
-        publish_wait();
         int32_t *top = done + DONE_SUB * DONE_STRIDE;
         const int n_top = ns < DONE_SUB ? ns : DONE_SUB;
         last = atomicAdd(top, 1) == n_top - 1;
@@ -644,7 +643,6 @@ __global__ __launch_bounds__(NT) void spg_pass_1d(PassArgs A) {
                 const int g = slot % DONE_SUB;
 #pragma unroll
                 for (int i = 0; i < NPART; i++) store_coherent(o + i, acc[i]);
-                publish_wait();
                 if (count_slot(A.done, g, uni(A.desc[inst].ntiles))) groups_done |= 1ull << g;
             } else {
 #pragma unroll

@@ -326,10 +326,11 @@ def run_graph_c5(args, ctx):
     pass_s, npass = _profiled_pass(P, params, True)
     P.close()
     R = w * h
-    # the fused image pass reads gamma_{i-1} and d_{i-1} (K - 1 rows each; the last rows follow
-    # from the simplex) and the data term q (K rows), and writes gamma_i and d_i:
-    # (4 (K - 1) + K) R * 8 bytes per iteration
-    own = (4.0 * (K - 1) + K) * R * 8
+    # the fused image pass reads p_{i-1} (K - 1 rows; the last row follows from the simplex)
+    # and the data term q (K rows), and writes gamma_i and p_i (K - 1 rows each), on the passes
+    # with beta_{i-1} = 1 (most of them; the others also read gamma_{i-1}):
+    # (3 (K - 1) + K) R * 8 bytes per iteration
+    own = (3.0 * (K - 1) + K) * R * 8
     return dict(workload=f"gamma QP solve, 1024x1024 RGB image, 4-neighbour grid, K={K}, xdim=3, "
                          f"epsilon {args.c5_eps:g}, theta fixed, gamma0=1/K, KKT<={args.kkt:g}",
                 time_s=float(np.mean(times)), iterations=int(r.it), kkt=float(r.kkt),

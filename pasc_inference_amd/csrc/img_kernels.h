@@ -9,23 +9,25 @@
 // neighbours come from the adjacent lanes (DPP lane shifts), the row above and below from the
 // registers of the walk, so each wave reads its rows once and needs neither LDS nor barriers.
 // Per image row y (even bands walk down, odd bands up) the wave
-//   * rebuilds gamma_i on the next row of the walk (gamma_{i-1} + beta_{i-1} d_{i-1}, K - 1
-//     stored rows),
-//   * forms g_i = alpha L gamma_i + q, d_i = P(gamma_i - alpha_i g_i) - gamma_i and, on passes
-//     that want it, the natural KKT residual on row y,
-//   * stores gamma_i and d_i (K - 1 rows) at its owned nodes and adds |d|^2, <g, d>, f and the
+//   * rebuilds gamma_i on the next row of the walk: gamma_{i-1} + beta_{i-1} (p_{i-1} -
+//     gamma_{i-1}) from the K - 1 stored rows of both, or, when beta_{i-1} = 1 (most passes),
+//     p_{i-1} itself, without reading gamma_{i-1},
+//   * forms g_i = alpha L gamma_i + q, p_i = P(gamma_i - alpha_i g_i), d_i = p_i - gamma_i and,
+//     on passes that want it, the natural KKT residual on row y,
+//   * stores gamma_i and p_i (K - 1 rows) at its owned nodes and adds |d|^2, <g, d>, f and the
 //     residual of those nodes,
 //   * adds alpha (d_a - d_b)^2 for the right edges of row y and the vertical edges between row
 //     y and the row before it in the walk (each vertical edge counted by the tile of its upper
 //     end), one row past the tile (the first row of the tile below) being formed for the last
 //     vertical edges.
 // The data term q (K rows, constant while theta is) is formed once per phase by img_data_term.
-// The raw rows of gamma_{i-1}, d_{i-1} and q are loaded PD rows before gamma_i is formed on
-// them.  Plain stores: nontemporal ones measured slower, although no lane reads gamma_i or d_i
+// The raw rows of gamma_{i-1}, p_{i-1} and q are loaded PD rows before gamma_i is formed on
+// them.  Plain stores: nontemporal ones measured slower, although no lane reads gamma_i or p_i
 // again in the pass.
-// Every node's gamma_i, Laplacian (walk order up, left, right, down, a missing neighbour's term
-// multiplied by 0) and d_i are those of the node-wise graph kernels; the sums over the nodes
-// run in the tile order, so they differ from those kernels' in rounding only.
+// Every node's Laplacian (walk order up, left, right, down, a missing neighbour's term
+// multiplied by 0) and projection are those of the node-wise graph kernels; gamma_i = p_{i-1}
+// when beta_{i-1} = 1 and the sums over the nodes in tile order differ from those kernels' in
+// rounding only.
 #pragma once
 
 #include "spg_kernels.h"
@@ -37,7 +39,7 @@ constexpr int IMG_OWN = WAVE - 3;  // owned columns of a wave tile
 struct ImgArgs {
     void *gamma0;
     void *gamma1;
-    void *dir0;  // d_{i-1} / d_i ping-pong, indexed like gamma by the instance parity
+    void *dir0;  // p_{i-1} / p_i ping-pong, indexed like gamma by the instance parity
     void *dir1;
     const void *q;  // data term rows, laid out like gamma
     const InstDesc *desc;
@@ -74,15 +76,22 @@ __global__ __launch_bounds__(256) void img_data_term(const InstDesc *desc, int n
     }
 }
 
-// The loads of one image row at this lane's column: K rows of gamma_0 (FIRST) or the K - 1
-// stored rows of gamma_{i-1} and d_{i-1}, and the K rows of the data term.
-template <typename S, int K, bool FIRST>
+// How a pass forms gamma_i from the previous pass's rows: IMG_FIRST projects gamma_0 (K rows);
+// IMG_BETA1 takes gamma_i = p_{i-1}, the previous pass's projected point (beta_{i-1} = 1, the
+// common case: gamma_{i-1} is not read); IMG_GENERAL forms gamma_{i-1} + beta_{i-1} (p_{i-1} -
+// gamma_{i-1}).
+constexpr int IMG_FIRST = 0, IMG_BETA1 = 1, IMG_GENERAL = 2;
+
+// The loads of one image row at this lane's column: the rows of gamma_{i-1} and p_{i-1} the mode
+// needs (K - 1 stored rows each; K rows of gamma_0 for IMG_FIRST) and the K rows of the data
+// term.
+template <typename S, int K, int MODE>
 struct ImgRaw {
     static constexpr int KS = stored_rows<K>();
-    static constexpr int NG = FIRST ? K : KS;
-    static constexpr int ND = FIRST ? 1 : KS;
-    S g[NG];
-    S d[ND];
+    static constexpr int NG = MODE == IMG_FIRST ? K : MODE == IMG_GENERAL ? KS : 0;
+    static constexpr int NP = MODE == IMG_FIRST ? 0 : KS;
+    S g[NG > 0 ? NG : 1];
+    S p[NP > 0 ? NP : 1];
     S q[K];
 };
 
@@ -91,8 +100,8 @@ template <typename S, typename C, int K>
 struct ImgCtx {
     const S *src;    // gamma_{i-1}
     S *dst;          // gamma_i
-    const S *dprev;  // d_{i-1}
-    S *dout;         // d_i
+    const S *pprev;  // p_{i-1} = P(gamma_{i-1} - alpha_{i-1} g_{i-1})
+    S *pout;         // p_i
     const S *q;      // data term
     uint32_t ld;     // K * ld * sizeof(S) < 4 GiB (engine_is_image)
     C alpha, a_dir, s_kkt, beta;
@@ -108,8 +117,8 @@ __device__ __forceinline__ void make_img_ctx(const ImgArgs &A, const InstDesc &D
     c.ld = (uint32_t)uni(D.ld);
     c.src = reinterpret_cast<const S *>(par ? A.gamma1 : A.gamma0) + goff;
     c.dst = reinterpret_cast<S *>(par ? A.gamma0 : A.gamma1) + goff;
-    c.dprev = reinterpret_cast<const S *>(par ? A.dir1 : A.dir0) + goff;
-    c.dout = reinterpret_cast<S *>(par ? A.dir0 : A.dir1) + goff;
+    c.pprev = reinterpret_cast<const S *>(par ? A.dir1 : A.dir0) + goff;
+    c.pout = reinterpret_cast<S *>(par ? A.dir0 : A.dir1) + goff;
     c.q = reinterpret_cast<const S *>(A.q) + goff;
     c.alpha = C(uni(D.alpha));
     c.alpha_d = uni(D.alpha);
@@ -134,32 +143,36 @@ __device__ __forceinline__ T &at32(T *base, uint32_t i) {
                                   (uint32_t)(i * (uint32_t)sizeof(T)));
 }
 
-template <typename S, typename C, int K, bool FIRST>
+template <typename S, typename C, int K, int MODE>
 __device__ __forceinline__ void img_load(const ImgCtx<S, C, K> &c, uint32_t n,
-                                         ImgRaw<S, K, FIRST> &r) {
-    using R = ImgRaw<S, K, FIRST>;
+                                         ImgRaw<S, K, MODE> &r) {
+    using R = ImgRaw<S, K, MODE>;
     const uint32_t ld = c.ld;
 #pragma unroll
     for (int k = 0; k < R::NG; k++) r.g[k] = at32(c.src, k * ld + n);
-    if constexpr (!FIRST) {
 #pragma unroll
-        for (int k = 0; k < R::ND; k++) r.d[k] = at32(c.dprev, k * ld + n);
-    }
+    for (int k = 0; k < R::NP; k++) r.p[k] = at32(c.pprev, k * ld + n);
 #pragma unroll
     for (int k = 0; k < K; k++) r.q[k] = at32(c.q, k * ld + n);
 }
 
-// gamma_i from the raw row (grid_gamma_u's arithmetic)
-template <typename S, typename C, int K, bool FIRST>
-__device__ __forceinline__ void img_gamma(const ImgRaw<S, K, FIRST> &r, C beta, C (&v)[K]) {
-    if constexpr (FIRST) {
+// gamma_i from the raw row (IMG_GENERAL: grid_gamma_u's arithmetic, with d_{i-1} = p_{i-1} -
+// gamma_{i-1} formed in registers)
+template <typename S, typename C, int K, int MODE>
+__device__ __forceinline__ void img_gamma(const ImgRaw<S, K, MODE> &r, C beta, C (&v)[K]) {
+    constexpr int KS = stored_rows<K>();
+    if constexpr (MODE == IMG_FIRST) {
 #pragma unroll
         for (int k = 0; k < K; k++) v[k] = C(r.g[k]);
         project_simplex<K, C>(v);
     } else {
-        constexpr int KS = stored_rows<K>();
+        if constexpr (MODE == IMG_BETA1) {
 #pragma unroll
-        for (int k = 0; k < KS; k++) v[k] = C(r.g[k]) + beta * C(r.d[k]);
+            for (int k = 0; k < KS; k++) v[k] = C(r.p[k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < KS; k++) v[k] = C(r.g[k]) + beta * (C(r.p[k]) - C(r.g[k]));
+        }
         if (KS < K) v[K - 1] = last_row(v);
     }
 }
@@ -168,13 +181,13 @@ __device__ __forceinline__ void img_gamma(const ImgRaw<S, K, FIRST> &r, C beta, 
 // two adjacent bands reach the rows they share at the same time (both at the end of their
 // walks, or both at the start), while those rows are still in L2.  acc[2] collects
 // sum(|d|^2 + off) and acc[3] sum((g + q) gamma); the caller scales them.
-template <typename S, typename C, int K, bool FIRST, int PD, bool DOWN>
+template <typename S, typename C, int K, int MODE, int PD, bool DOWN>
 __device__ __forceinline__ void img_tile(const ImgCtx<S, C, K> &c, int slot, int W, int H,
                                          int rows, int ncol, double (&acc)[NPART]) {
     constexpr int KS = stored_rows<K>();
     constexpr int NB = PD + 1;
     constexpr int STEP = DOWN ? 1 : -1;
-    using Raw = ImgRaw<S, K, FIRST>;
+    using Raw = ImgRaw<S, K, MODE>;
     const int lane = threadIdx.x & (WAVE - 1);
     const int band = slot / ncol, ct = slot - band * ncol;
     const int col = ct * IMG_OWN - 1 + lane;
@@ -200,13 +213,13 @@ __device__ __forceinline__ void img_tile(const ImgCtx<S, C, K> &c, int slot, int
     for (int k = 0; k < K; k++) { gB[k] = C(0); dB[k] = C(0); gA[k] = C(0); }
     if (yb >= 0 && yb < H) {
         Raw behind;
-        img_load<S, C, K, FIRST>(c, node(yb), behind);
-        img_gamma<S, C, K, FIRST>(behind, c.beta, gB);
+        img_load<S, C, K, MODE>(c, node(yb), behind);
+        img_gamma<S, C, K, MODE>(behind, c.beta, gB);
     }
     Raw buf[NB];
 #pragma unroll
-    for (int i = 0; i < NB; i++) img_load<S, C, K, FIRST>(c, node(ys + i * STEP), buf[i]);
-    img_gamma<S, C, K, FIRST>(buf[0], c.beta, gC);
+    for (int i = 0; i < NB; i++) img_load<S, C, K, MODE>(c, node(ys + i * STEP), buf[i]);
+    img_gamma<S, C, K, MODE>(buf[0], c.beta, gC);
 
     const int nsteps = ylast - r0 + 1;
     for (int i0 = 0; i0 < nsteps; i0 += NB) {
@@ -218,9 +231,9 @@ __device__ __forceinline__ void img_tile(const ImgCtx<S, C, K> &c, int slot, int
             C q[K];
 #pragma unroll
             for (int k = 0; k < K; k++) q[k] = C(buf[u].q[k]);
-            img_load<S, C, K, FIRST>(c, node(y + NB * STEP), buf[u]);
+            img_load<S, C, K, MODE>(c, node(y + NB * STEP), buf[u]);
             const int ya = y + STEP;
-            if (ya >= 0 && ya < H) img_gamma<S, C, K, FIRST>(buf[(u + 1) % NB], c.beta, gA);
+            if (ya >= 0 && ya < H) img_gamma<S, C, K, MODE>(buf[(u + 1) % NB], c.beta, gA);
             const C wu = y > 0 ? C(1) : C(0), wd = y + 1 < H ? C(1) : C(0);
             // Laplacian in the walk order up, left, right, down
             C lap[K];
@@ -252,7 +265,7 @@ __device__ __forceinline__ void img_tile(const ImgCtx<S, C, K> &c, int slot, int
 #pragma unroll
                 for (int k = 0; k < KS; k++) {
                     at32(c.dst, k * ld + n) = S(gC[k]);
-                    at32(c.dout, k * ld + n) = S(d[k]);
+                    at32(c.pout, k * ld + n) = S(p[k]);
                 }
             }
             const double wn = own ? 1.0 : 0.0;
@@ -326,13 +339,17 @@ __global__ __launch_bounds__(WAVE) void spg_pass_img(ImgArgs A) {
         make_img_ctx<S, C, K>(A, D, st, cx);
         double acc[NPART] = {0.0, 0.0, 0.0, 0.0, cx.want_kkt ? 0.0 : -1.0};
         const bool down = ((slot / ncol) & 1) == 0;
-        if (uni(st.first)) {
-            if (down) img_tile<S, C, K, true, PD, true>(cx, slot, W, H, rows, ncol, acc);
-            else img_tile<S, C, K, true, PD, false>(cx, slot, W, H, rows, ncol, acc);
-        } else {
-            if (down) img_tile<S, C, K, false, PD, true>(cx, slot, W, H, rows, ncol, acc);
-            else img_tile<S, C, K, false, PD, false>(cx, slot, W, H, rows, ncol, acc);
-        }
+        const int mode = uni(st.first) ? IMG_FIRST
+                                       : (uni(st.beta_apply) == 1.0 ? IMG_BETA1 : IMG_GENERAL);
+#define PASC_IMG_TILE(M)                                                                   \
+    do {                                                                                   \
+        if (down) img_tile<S, C, K, M, PD, true>(cx, slot, W, H, rows, ncol, acc);         \
+        else img_tile<S, C, K, M, PD, false>(cx, slot, W, H, rows, ncol, acc);             \
+    } while (0)
+        if (mode == IMG_FIRST) PASC_IMG_TILE(IMG_FIRST);
+        else if (mode == IMG_BETA1) PASC_IMG_TILE(IMG_BETA1);
+        else PASC_IMG_TILE(IMG_GENERAL);
+#undef PASC_IMG_TILE
         // <g, d> in the sign-safe form of the projected-gradient step (pass_tile_1d), f = 1/2
         // <g + q, gamma>
         acc[2] = -acc[2] / cx.a_dir_d;

@@ -238,3 +238,33 @@ def test_image_autotune_keeps_iterate(ctx):
         r = P.gamma_solve(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=1e-10)
         assert r.stop_reason == capi.REASON_KKT
         P.close()
+
+
+@pytest.mark.parametrize("K", [1, 2, 5, 8])
+def test_image_pass_cluster_counts_match_oracle(ctx, oracle, K):
+    """The image pass at the smallest and largest cluster counts (K = 1: gamma fixed at 1; K = 8:
+    the largest register footprint), RGB data, FP64 and FP32-bulk, against the oracle."""
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Problem, generate_rect_image
+    w, h, eps = 97, 31, 2.0
+    x, colr, _ = generate_rect_image(500 + K, w, h, K, 9, 0.2)
+    R = w * h
+    b = oracle.assemble_b(x, colr, 1, R, K, xdim=3)
+    xs, rr = oracle.spg_solve(np.full(R * K, 1.0 / K), b, 1, R, K, eps, graph=grid_csr(w, h),
+                              maxit=200000, stop_difff=0, stop_kkt=1, kkt_tol=1e-12,
+                              kkt_scale=1.0, nthreads=4, gd_centered=1, gd_certified=1,
+                              dad_edges=1, g_recompute=1)
+    for prec in (capi.FP64, capi.FP32_BULK):
+        P = Problem(ctx, x, K, T=1, xdim=3, grid_width=w, precision=prec, epssqr=eps)
+        P.set_theta(colr)
+        P.set_gamma_uniform()
+        print(f"image pass K={K} prec {prec}", flush=True)
+        r = P.gamma_solve(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=1e-11)
+        g = P.get_gamma()
+        f, kkt = P.objective()
+        print(f"  it={r.it} kkt={kkt:.2e} f={f!r} oracle f={rr.fx!r} "
+              f"err={np.max(np.abs(g - xs)):.2e}", flush=True)
+        assert r.status == 0
+        assert abs(f - rr.fx) <= 1e-8 * max(abs(rr.fx), 1e-12)
+        assert np.max(np.abs(g - xs)) <= 1e-6
+        P.close()

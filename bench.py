@@ -310,7 +310,8 @@ def run_graph_c5(args, ctx):
     """The graph stencil at the image configuration: a 1024 x 1024 RGB image on the 4-neighbour
     grid (T = 1 step of R = 1048576 nodes, K = 4, xdim = 3; 64 random rectangles, noise 0.15,
     seed 4), theta at the generating colours, one FP64 gamma solve from gamma0 = 1/K to the
-    bench's KKT target.  Reported beside the headline, not as it."""
+    bench's KKT target, and the same solve with FP32 bulk iterations and an FP64 polish.
+    Reported beside the headline, not as it."""
     from pasc_inference_amd import capi
     from pasc_inference_amd.solver import Problem, generate_rect_image
     w = h = 1024
@@ -324,7 +325,22 @@ def run_graph_c5(args, ctx):
     g = P.get_gamma().reshape(-1, K)
     acc = float(np.mean(g.argmax(1) == lab))
     pass_s, npass = _profiled_pass(P, params, True)
+    g64 = P.get_gamma()
     P.close()
+    # the same solve with the image's FP32 bulk (FP32 iterations, then an FP64 polish from the
+    # FP32 iterate to the same KKT target)
+    P32 = Problem(ctx, x, K, T=1, xdim=3, grid_width=w, precision=capi.FP32_BULK,
+                  epssqr=args.c5_eps)
+    P32.set_theta(col.ravel().copy())
+    times32, r32 = _timed_solves(args, P32, params, "graph (image) fp32-bulk solve")
+    g32 = P32.get_gamma()
+    P32.set_profiling(True)
+    P32.set_gamma_uniform()
+    rp = P32.gamma_solve(**params)
+    P32.set_profiling(False)
+    pass32_s = rp.pass_time_fp32_s / max(rp.passes_fp32, 1)
+    polish_s = rp.pass_time_fp64_s / max(rp.passes_fp64, 1)
+    P32.close()
     R = w * h
     # the fused image pass reads p_{i-1} (K - 1 rows; the last row follows from the simplex)
     # and the data term q (K rows), and writes gamma_i and p_i (K - 1 rows each), on the passes
@@ -337,6 +353,11 @@ def run_graph_c5(args, ctx):
                 us_per_iteration=float(np.mean(times)) / (r.it + 1) * 1e6,
                 label_accuracy=acc, pass_us=pass_s * 1e6, own_min_gbps=own / pass_s / 1e9,
                 kernels="spg_pass_img (one launch per iteration, step in its last workgroup)",
+                fp32_bulk=dict(time_s=float(np.mean(times32)), it_fp32=int(r32.it_fp32),
+                               it_fp64=int(r32.it_fp64), kkt=float(r32.kkt),
+                               pass_us_fp32=pass32_s * 1e6, pass_us_fp64=polish_s * 1e6,
+                               own_min_gbps_fp32=own / 2 / pass32_s / 1e9,
+                               gamma_maxabs_vs_fp64=float(np.abs(g32 - g64).max())),
                 note="the iteration count depends on the image rows per wave tile (the order "
                      "of the FP64 sums), so pass_us is the figure to compare")
 

@@ -202,6 +202,12 @@ int pasc_gamma_solve(pasc_problem *p, const pasc_spg_params *params, pasc_spg_re
  * explicit b. */
 int pasc_set_linear_term(pasc_problem *p, const double *b, int layout);
 
+/* ---- replace the data: the datavector edits of TSData::cutdata / scaledata / unscaledata
+ * (tsdata.cpp:384-478,511-545) made on the host and uploaded.  x has the layout and size of
+ * pasc_problem_desc.x ([T][R][xdim], finite); theta, gamma and epssqr are kept, and later solves
+ * and theta steps use the new data. */
+int pasc_set_data(pasc_problem *p, const double *x);
+
 /* ---- the theta step: updatebeforesolve_thetasolver (graphh1fem.cpp:305-396), FP64 sums ---- */
 int pasc_theta_update(pasc_problem *p);
 
@@ -317,7 +323,8 @@ int pasc_device_copy_bandwidth(pasc_ctx *ctx, int64_t bytes, int32_t reps, doubl
 
 /* ---- launch shape of the fused 1-D pass and of the image pass ----
  * Images (T = 1 on the implicit 4-neighbour grid) run the fused image pass: block must be 64,
- * chunk is the number of image rows per one-wave tile (1..256; default 12) and prefetch the
+ * chunk is the number of image rows per one-wave tile (1..256; default: the rows in 6..24 that
+ * spread the tiles most evenly over the SIMDs, 18 on a 1024 x 1024 image) and prefetch the
  * number of raw rows loaded ahead besides the next one (1..3, 0 = 1); max_slots and
  * min_tiles are not used.  Other graphs have a fixed shape. */
 typedef struct {
@@ -341,9 +348,9 @@ int pasc_set_launch_shape(pasc_problem *p, const pasc_launch_shape *shape);
 
 /* ---- autotuner ----
  * Times the fused SPG pass for the problem's (K, precision, T) over the candidate launch
- * shapes (block x chunk x prefetch x slots; images: 6, 8, 12, 16, 24 rows per tile) and keeps
- * the fastest for later solves; for an FP32-bulk problem the FP32 phase and the FP64 polish
- * are tuned separately.
+ * shapes (block x chunk x prefetch x slots; images: the default rows and 6, 8, 12, 16, 24 rows
+ * per tile) and keeps the fastest for later solves; for an FP32-bulk problem the FP32 phase and
+ * the FP64 polish are tuned separately.
  * The choice is cached per context.  With a non-NULL cache_path, a choice recorded there for
  * the same key by an earlier process is reused without timing, and every call appends its
  * choice to the file (one JSON object per line). */

@@ -67,6 +67,29 @@ static int img_nrow(const Engine &e, int rows) {
     return (int)((h + rows - 1) / rows);
 }
 
+// Image rows per wave tile that spread the one-wave tiles evenly over the SIMDs (4 per CU).  A
+// SIMD that holds m tiles walks m x (rows + 1) image rows (the last one forms the halo row),
+// and the pass lasts about as long as the longest such walk: the rows in [6, 24] with the least
+// m (rows + 1), the most rows on a tie (fewer partial slots).  On the 1024 x 1024 image (17
+// column tiles, 1024 SIMDs) that is 18 rows (969 tiles, cost 19); at a fixed 600 FP64 passes
+// 18 / 9 / 10 / 11 / 12 rows (cost 19 / 20 / 22 / 24 / 26) took 27.5 / 27.7 / 27.9 / 28.4 /
+// 28.6 us per pass, the order of the costs.  Small images (at most one tile per SIMD at any
+// rows) get 6 rows: shorter walks.
+static int img_balanced_rows(const Engine &e) {
+    const int64_t nsimd = (int64_t)std::max(e.ctx->n_cu, 1) * 4;
+    int best = 6;
+    int64_t best_cost = std::numeric_limits<int64_t>::max();
+    for (int r = 6; r <= 24; r++) {
+        const int64_t tiles = (int64_t)img_ncol(e) * img_nrow(e, r);
+        const int64_t cost = (tiles + nsimd - 1) / nsimd * (r + 1);
+        if (cost <= best_cost) {
+            best_cost = cost;
+            best = r;
+        }
+    }
+    return best;
+}
+
 static int launch_pass_1d(bool f64, int K, const PassArgs &a, int grid, int block, int chunk,
                           int pf, hipStream_t s) {
     if (f64) {
@@ -299,10 +322,13 @@ int engine_set_graph(Engine &e, int64_t T, int64_t R, int32_t width, const int64
         e.geo.idx = e.d_nbr_idx;
         e.geo.width = 0;
     }
-    // images: one-wave tiles of 12 image rows, one row loaded ahead, in both phases (on the
-    // 1024 x 1024 RGB image, K = 4: FP64 31.6 us per iteration against 32.9 / 33.0 us for 8 /
-    // 16 rows; a second row ahead measured 0.5 - 2 us slower); set_launch_shape changes them
-    if (engine_is_image(e)) e.sh[0] = e.sh[1] = PassShape{WAVE, 12, 1, 0, 1};
+    // images: one-wave tiles of img_balanced_rows image rows, one row loaded ahead (a second
+    // row ahead measured 0.1 - 0.8 us slower on the 1024 x 1024 image), in both phases;
+    // set_launch_shape changes them
+    if (engine_is_image(e)) {
+        const int rows = img_balanced_rows(e);
+        e.sh[0] = e.sh[1] = PassShape{WAVE, rows, 1, 0, 1};
+    }
     return PASC_OK;
 }
 
@@ -1177,9 +1203,10 @@ int engine_set_shape(Engine &e, const PassShape &s32_in, const PassShape &s64_in
 // and 20 FP64 polish passes per candidate, and the fastest shape of each phase is kept (the
 // preferred shape when it is within 5% of the fastest, see below).  The iterate is restored
 // afterwards.
-// Images: times 20 passes of the image pass (per phase) for 6, 8, 12, 16 and 24 image rows per
-// wave tile and keeps the fastest, or 12 rows when within 5% of it (the rows fix the order of
-// the sums, so a choice that flips between near-equal shapes would change the iterates).
+// Images: times 20 passes of the image pass (per phase) for the balanced rows
+// (img_balanced_rows) and 6, 8, 12, 16 and 24 image rows per wave tile and keeps the fastest, or
+// the balanced rows when within 5% of it (the rows fix the order of the sums, so a choice that
+// flips between near-equal shapes would change the iterates).
 static int img_autotune(Engine &e, const std::vector<std::vector<double>> &saved,
                         PassShape *s32, PassShape *s64, double us[2]) {
     const bool f32 = e.precision == PASC_FP32_BULK;
@@ -1191,11 +1218,13 @@ static int img_autotune(Engine &e, const std::vector<std::vector<double>> &saved
     p.kkt_every = 4;
     const bool prof = e.profile;
     e.profile = true;
-    constexpr int PREF = 12;
-    const int rows[5] = {6, 8, 12, 16, 24};
+    const int PREF = img_balanced_rows(e);
+    std::vector<int> rows = {PREF};
+    for (int r : {6, 8, 12, 16, 24})
+        if (r != PREF) rows.push_back(r);
     double best[2] = {1e300, 1e300}, pref_t[2] = {1e300, 1e300};
     int best_rows[2] = {PREF, PREF};
-    for (int ri = 0; ri < 5; ri++) {
+    for (size_t ri = 0; ri < rows.size(); ri++) {
         e.sh[0] = e.sh[1] = PassShape{WAVE, rows[ri], 1, 0, 1};
         int rc = engine_retile(e, !f32);
         for (int i = 0; i < e.n_inst && rc == PASC_OK; i++) rc = upload_rows(e, i, saved[i]);

@@ -207,6 +207,41 @@ def test_image_pass_tiles_match_oracle(ctx, oracle, w, h, rows):
         P.close()
 
 
+@pytest.mark.parametrize("K,xdim", [(4, 3), (3, 1), (3, 2), (8, 3)])
+def test_image_pass_channels_and_heldout_match_oracle(ctx, oracle, K, xdim):
+    """The image pass on grayscale, two-channel and RGB data (xdim < K) with 15% of the pixels
+    held out (data term 0 there), FP64 and FP32-bulk, against the oracle's minimiser of the same
+    QP on the explicit 4-neighbour graph."""
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Problem, generate_mask, generate_rect_image
+    w, h, eps = 75, 44, 2.0
+    x3, col3, _ = generate_rect_image(900 + 10 * K + xdim, w, h, K, 8, 0.2)
+    R = w * h
+    x = x3.reshape(R, 3)[:, :xdim].ravel().copy()
+    colr = col3.reshape(K, 3)[:, :xdim].ravel().copy()
+    mask = generate_mask(77 + K, R, 0.15)
+    b = oracle.assemble_b(x, colr, 1, R, K, xdim=xdim, heldout=mask)
+    xs, rr = oracle.spg_solve(np.full(R * K, 1.0 / K), b, 1, R, K, eps, graph=grid_csr(w, h),
+                              maxit=200000, stop_difff=0, stop_kkt=1, kkt_tol=1e-12,
+                              kkt_scale=1.0, nthreads=4, gd_centered=1, gd_certified=1,
+                              dad_edges=1, g_recompute=1)
+    for prec in (capi.FP64, capi.FP32_BULK):
+        P = Problem(ctx, x, K, T=1, xdim=xdim, grid_width=w, precision=prec, heldout=mask,
+                    epssqr=eps)
+        P.set_theta(colr)
+        P.set_gamma_uniform()
+        print(f"image pass K={K} xdim={xdim} prec {prec}", flush=True)
+        r = P.gamma_solve(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=1e-11)
+        g = P.get_gamma()
+        f, kkt = P.objective()
+        print(f"  it={r.it} kkt={kkt:.2e} f={f!r} oracle f={rr.fx!r} "
+              f"err={np.max(np.abs(g - xs)):.2e}", flush=True)
+        assert r.stop_reason == capi.REASON_KKT
+        assert abs(f - rr.fx) <= 1e-8 * abs(rr.fx)
+        assert np.max(np.abs(g - xs)) <= 1e-6
+        P.close()
+
+
 def test_image_pass_rejects_bad_shapes(ctx):
     from pasc_inference_amd import capi
     from pasc_inference_amd.solver import Problem
@@ -233,7 +268,7 @@ def test_image_autotune_keeps_iterate(ctx):
         sh = P.autotune()
         print(f"image autotune prec {prec}: {sh}", flush=True)
         for ph in ("main", "polish"):
-            assert sh[ph]["block"] == 64 and sh[ph]["chunk"] in (6, 8, 12, 16, 24)
+            assert sh[ph]["block"] == 64 and 6 <= sh[ph]["chunk"] <= 24
         assert np.array_equal(P.get_gamma(), g0)
         r = P.gamma_solve(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=1e-10)
         assert r.stop_reason == capi.REASON_KKT

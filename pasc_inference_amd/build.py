@@ -69,22 +69,25 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     return LIB
 
 
+HOST_DRIVERS = ("test_signal1d", "test_image")
 HOST_DRIVER = os.path.join(HERE, "host", "test_signal1d")
 
 
 def build_host(verbose: bool = False) -> str:
-    """The C++ host layer's driver (host/examples/test_signal1d.cpp), linked to the library."""
+    """The C++ host layer's drivers (host/examples/*.cpp), linked to the library."""
     gxx = shutil.which("g++") or "g++"
     root = os.path.dirname(HERE)
-    cmd = [gxx, "-std=c++17", "-O2", "-Wall", "-Wextra",
-           "-I", os.path.join(root, "include"), "-I", os.path.join(HERE, "host", "include"),
-           os.path.join(HERE, "host", "examples", "test_signal1d.cpp"),
-           "-L", HERE, "-lpasc_hip", "-Wl,-rpath,$ORIGIN/..", "-o", HOST_DRIVER]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"host driver build failed:\n{r.stderr[-6000:]}")
-    if verbose and r.stderr.strip():
-        print(r.stderr)
+    for name in HOST_DRIVERS:
+        cmd = [gxx, "-std=c++17", "-O2", "-Wall", "-Wextra",
+               "-I", os.path.join(root, "include"), "-I", os.path.join(HERE, "host", "include"),
+               os.path.join(HERE, "host", "examples", name + ".cpp"),
+               "-L", HERE, "-lpasc_hip", "-Wl,-rpath,$ORIGIN/..",
+               "-o", os.path.join(HERE, "host", name)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"host driver {name} build failed:\n{r.stderr[-6000:]}")
+        if verbose and r.stderr.strip():
+            print(r.stderr)
     return HOST_DRIVER
 
 

@@ -118,6 +118,7 @@ _SIGS = {
     "pasc_gamma_solve": ([vp, C.POINTER(SpgParams), C.POINTER(SpgResult)], C.c_int),
     "pasc_theta_update": ([vp], C.c_int),
     "pasc_set_linear_term": ([vp, C.POINTER(C.c_double), C.c_int], C.c_int),
+    "pasc_set_data": ([vp, C.POINTER(C.c_double)], C.c_int),
     "pasc_fem_reduced_T": ([C.c_int64, C.c_double], C.c_int64),
     "pasc_fem_reduce": ([vp, vp, vp, C.c_int64, C.c_int64, C.c_int32, C.c_int64, C.c_int64,
                          C.c_int32], C.c_int),

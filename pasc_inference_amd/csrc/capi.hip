@@ -545,6 +545,33 @@ int pasc_set_linear_term(pasc_problem *p, const double *b, int layout) {
     return rc;
 }
 
+int pasc_set_data(pasc_problem *p, const double *x) {
+    clear_error();
+    if (!p || !x) return fail(PASC_ERR_ARG, "set_data: NULL argument");
+    if (!p->has_data) return fail(PASC_ERR_STATE, "set_data: the problem was created without data");
+    const int64_t n = p->T * p->R;
+    for (int64_t i = 0; i < n * p->xdim; i++)
+        if (!std::isfinite(x[i])) return fail(PASC_ERR_ARG, "set_data: x[%lld] is not finite", (long long)i);
+    int rc = use_device(p->ctx);
+    if (rc) return rc;
+    // the data rows as at creation: SoA, row c holds channel c (the fine grid when FEM-reduced)
+    const int64_t ld = p->fem_type != PASC_FEM_NONE ? p->fg.ld1 : p->e.h_desc[0].ld;
+    std::vector<double> rows((size_t)ld * p->xdim, 0.0);
+    for (int c = 0; c < p->xdim; c++)
+        for (int64_t i = 0; i < n; i++) rows[(size_t)c * ld + i] = x[i * p->xdim + c];
+    if (p->fem_type != PASC_FEM_NONE) {
+        const hipError_t he = hipMemcpy(p->d_x1, rows.data(), sizeof(double) * rows.size(),
+                                        hipMemcpyHostToDevice);
+        if (he != hipSuccess) return fail(PASC_ERR_HIP, "set_data: %s", hipGetErrorString(he));
+    } else {
+        rc = engine_upload_data(p->e, rows.data(), 0, (int64_t)rows.size());
+        if (rc) return rc;
+    }
+    p->h_x.assign(x, x + n * p->xdim);
+    p->solved = false;
+    return PASC_OK;
+}
+
 int pasc_theta_update(pasc_problem *p) {
     clear_error();
     if (!p) return fail(PASC_ERR_ARG, "theta_update: NULL problem");

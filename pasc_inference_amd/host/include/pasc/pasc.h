@@ -3,6 +3,7 @@
 
 #include "common.h"
 #include "decomposition.h"
+#include "imagedata.h"
 #include "model.h"
 #include "solvers.h"
 #include "tsdata.h"

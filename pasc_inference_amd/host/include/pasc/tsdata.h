@@ -12,23 +12,27 @@ template <class VectorBase> class TSData;
 
 template <> class TSData<HipVector> {
   public:
-    // T time steps of xdim values (R = 1), K clusters, precision PASC_FP64 / PASC_FP32_BULK.
-    // fem_reduce < 1 solves the gamma QP on a coarse grid (the reference passes a Fem / FemHat
-    // object to GraphH1FEMModel, test_signal1D.cpp:218-228; here the problem owns it).
+    // T time steps of R nodes with xdim values each ([T][R][xdim]), K clusters, precision
+    // PASC_FP64 / PASC_FP32_BULK.  R > 1 with grid_width > 0: the nodes form the 4-neighbour
+    // grid of that width (the BGMGraphGrid2D of test_image.cpp:156-168).  fem_reduce < 1 solves
+    // the gamma QP on a coarse grid (the reference passes a Fem / FemHat object to
+    // GraphH1FEMModel, test_signal1D.cpp:218-228; here the problem owns it).
     TSData(pasc_ctx *ctx, const std::vector<double> &x, int K, int xdim = 1,
            int precision = PASC_FP64, const std::vector<uint8_t> *heldout = nullptr,
-           double fem_reduce = 1.0, int fem_type = PASC_FEM_HAT)
-        : ctx_(ctx), x_(x), K_(K), xdim_(xdim) {
-        T_ = (int64_t)x.size() / xdim;
+           double fem_reduce = 1.0, int fem_type = PASC_FEM_HAT, int64_t R = 1,
+           int32_t grid_width = 0)
+        : ctx_(ctx), x_(x), K_(K), xdim_(xdim), R_(R) {
+        T_ = (int64_t)x.size() / ((int64_t)xdim * R);
         pasc_problem_desc d;
         std::memset(&d, 0, sizeof(d));
         d.T = T_;
-        d.R = 1;
+        d.R = R;
         d.K = K;
         d.xdim = xdim;
         d.scalef = 1;
         d.precision = precision;
         d.x = x_.data();
+        d.grid_width = grid_width;
         d.fem_reduce = fem_reduce;
         d.fem_type = fem_type;
         if (heldout) {
@@ -47,14 +51,14 @@ template <> class TSData<HipVector> {
     bool ok() const { return ok_; }
     pasc_problem *get_problem() const { return problem_; }
     int64_t get_T() const { return T_; }
-    int get_R() const { return 1; }
+    int64_t get_R() const { return R_; }
     int get_K() const { return K_; }
     int get_xdim() const { return xdim_; }
     const std::vector<double> &get_datavector() const { return x_; }
 
     // gamma in the reference's t-major layout
     std::vector<double> get_gammavector() const {
-        std::vector<double> g((size_t)T_ * K_);
+        std::vector<double> g((size_t)T_ * R_ * K_);
         PASC_TRY(pasc_get_gamma(problem_, g.data(), PASC_LAYOUT_TMAJOR));
         return g;
     }
@@ -72,8 +76,58 @@ template <> class TSData<HipVector> {
     // load_gammavector (tsdata.cpp:548-627): k-major file permuted to t-major
     void load_gammavector(const std::string &filename) {
         const std::vector<double> g = petscio::read_vec(filename);
-        if ((int64_t)g.size() != T_ * K_) throw std::runtime_error("gamma0 has the wrong length");
+        if ((int64_t)g.size() != T_ * R_ * K_) throw std::runtime_error("gamma0 has the wrong length");
         PASC_TRY(pasc_set_gamma(problem_, g.data(), PASC_LAYOUT_KMAJOR));
+    }
+
+    // cutdata (tsdata.cpp:461-478): clamp the data to [a, b]
+    void cutdata(double a, double b) {
+        for (double &v : x_) v = v > b ? b : (v < a ? a : v);
+        PASC_TRY(pasc_set_data(problem_, x_.data()));
+    }
+    // scaledata (tsdata.cpp:384-422): map [min x, max x] linearly onto [a, b], theta with it
+    void scaledata(double a, double b) {
+        double lo = x_.empty() ? 0.0 : x_[0], hi = lo;
+        for (double v : x_) {
+            lo = v < lo ? v : lo;
+            hi = v > hi ? v : hi;
+        }
+        scaledata(a, b, lo, hi);
+    }
+    // scaledata (tsdata.cpp:511-545): map [scale_min, scale_max] linearly onto [a, b]
+    void scaledata(double a, double b, double scale_min, double scale_max) {
+        scale_min_ = scale_min;
+        scale_max_ = scale_max;
+        const double k = (b - a) / (scale_max - scale_min), q = a - k * scale_min;
+        for (double &v : x_) v = k * v + q;
+        PASC_TRY(pasc_set_data(problem_, x_.data()));
+        std::vector<double> th = get_thetavector();
+        for (double &v : th) v = k * v + q;
+        set_thetavector(th);
+    }
+    // unscaledata (tsdata.cpp:424-459): the inverse of the last scaledata(a, b)
+    void unscaledata(double a, double b) {
+        const double k = (b - a) / (scale_max_ - scale_min_), q = a - k * scale_min_;
+        for (double &v : x_) v = (v - q) / k;
+        PASC_TRY(pasc_set_data(problem_, x_.data()));
+        std::vector<double> th = get_thetavector();
+        for (double &v : th) v = (v - q) / k;
+        set_thetavector(th);
+    }
+    // cutgamma (tsdata.cpp:196-232): every node's gamma set to the indicator of its largest
+    // component (the first one on a tie).  The reference starts each node's search from
+    // gamma_arr[t] (tsdata.cpp:212), an element of another node for t > 0; here the search
+    // starts from the node's own component 0.
+    void cutgamma() {
+        std::vector<double> g = get_gammavector();
+        const size_t n = (size_t)(T_ * R_);
+        for (size_t i = 0; i < n; i++) {
+            int best = 0;
+            for (int k = 1; k < K_; k++)
+                if (g[i * K_ + k] > g[i * K_ + best]) best = k;
+            for (int k = 0; k < K_; k++) g[i * K_ + k] = k == best ? 1.0 : 0.0;
+        }
+        set_gammavector(g);
     }
     double get_aic() const { return aic_; }
     void set_aic(double a) { aic_ = a; }
@@ -89,8 +143,9 @@ template <> class TSData<HipVector> {
     std::vector<double> x_;
     std::vector<uint8_t> heldout_;
     std::vector<double> theta_;
-    int64_t T_ = 0;
+    int64_t T_ = 0, R_ = 1;
     int K_ = 0, xdim_ = 1;
+    double scale_min_ = 0.0, scale_max_ = 1.0;
     double aic_ = std::numeric_limits<double>::max();
     bool ok_ = false;
 };

@@ -154,6 +154,15 @@ class Problem:
         assert self._b.size == self.nodes * self.K
         check(lib().pasc_set_linear_term(self.h, dptr(self._b), layout))
 
+    def set_data(self, x):
+        """Replace the data (same layout and size as at creation), as TSData::cutdata /
+        scaledata edit the datavector; theta, gamma and epsilon are kept."""
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        assert x.size == self.x.size
+        check(lib().pasc_set_data(self.h, dptr(x)))
+        self.x = x
+        self._keep["x"] = x
+
     def theta_update(self):
         check(lib().pasc_theta_update(self.h))
         return self.get_theta()

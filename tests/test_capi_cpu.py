@@ -124,10 +124,11 @@ def test_null_arguments_are_rejected():
     assert (tp.maxit, tp.eps) == (300, 1e-6)  # tssolver.h:13-14
 
 
-def test_cpp_host_driver_is_built_and_fails_loudly_without_device():
+@pytest.mark.parametrize("name", ["test_signal1d", "test_image"])
+def test_cpp_host_driver_is_built_and_fails_loudly_without_device(name):
     import os
     import subprocess
-    drv = os.path.join(os.path.dirname(capi.HERE), "pasc_inference_amd", "host", "test_signal1d")
+    drv = os.path.join(os.path.dirname(capi.HERE), "pasc_inference_amd", "host", name)
     assert os.path.exists(drv)
     r = subprocess.run([drv], capture_output=True, text=True)
     assert r.returncode == 1 and "test_epssqr has to be set" in r.stdout

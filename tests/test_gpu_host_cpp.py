@@ -1,17 +1,20 @@
-"""The C++ host layer (TSSolver / GraphH1FEMModel / SPGQPSolver over the HipVector backend) run
-through its test_signal1d driver on the reference's own data fixture."""
+"""The C++ host layer (TSSolver / GraphH1FEMModel / SPGQPSolver / ImageData over the HipVector
+backend) run through its test_signal1d and test_image drivers on the reference's own data
+fixtures."""
 import os
 import re
 import subprocess
 
 import numpy as np
 import pytest
+from helpers import grid_csr
 
 pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 DRIVER = os.path.join(ROOT, "pasc_inference_amd", "host", "test_signal1d")
+IMAGE_DRIVER = os.path.join(ROOT, "pasc_inference_amd", "host", "test_image")
 DATA = os.path.join(HERE, "golden", "reference_data")
 
 
@@ -96,3 +99,57 @@ def test_cpp_driver_fem_reduced_scan(oracle, tmp_path):
             assert abs(a_dev - a_ref) <= 1e-4 * a_ref + 1e-3
         gk = petsc_io.read_vec(out + "_gamma.bin")
         assert gk.size == T * K  # the saved gamma is the fine (prolongated) one
+
+
+def test_cpp_image_driver_on_reference_image(oracle, tmp_path):
+    """test_image (the flow of examples/test_image/test_image.cpp) on the reference's own
+    250 x 150 noisy text image (examples/test_image/data/usi_text/usi_250_150_02.bin): data cut
+    to [0, 1], K = 2, a sorted epsilon list solved in turn from the previous gamma and theta.
+    Per epsilon the saved files match the oracle's alternation on the same data and 4-neighbour
+    graph: gamma (k-major, permute_TRK inverted) within 1e-6, theta (shortinfo, printed with 6
+    digits) within 1e-5, the recovered image sum_k theta_k gamma_k, the original image (cut) on
+    the first epsilon only, and one shortinfo line each."""
+    from pasc_inference_amd import petsc_io
+    assert os.path.exists(IMAGE_DRIVER), "build the host drivers first"
+    W, H, K = 250, 150, 2
+    R = W * H
+    img = os.path.join(DATA, "usi_250_150_02.bin")
+    eps_list = [0.3, 0.03, 3.0]  # sorted by the driver
+    res = str(tmp_path / "results")
+    short = str(tmp_path / "short" / "usi.txt")
+    cmd = [IMAGE_DRIVER, f"--test_image_filename={img}", f"--test_width={W}",
+           f"--test_height={H}", f"--test_K={K}", f"--test_results_dir={res}",
+           "--test_image_out=usi", f"--test_shortinfo_filename={short}",
+           "--spgqpsolver_stop_difff=0", "--spgqpsolver_stop_kkt=1",
+           "--spgqpsolver_kkt_tol=1e-11", "--spgqpsolver_maxit=100000"]
+    cmd += [f"--test_epssqr={e}" for e in eps_list]
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-3000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    # the oracle: cutdata(0, 1) (tsdata.cpp:461-478), theta0 spread over the data range, a
+    # uniform gamma0 and a gamma step first, each epsilon from the previous gamma and theta
+    x = np.clip(petsc_io.read_vec(img), 0.0, 1.0)
+    lo, hi = x.min(), x.max()
+    th = np.array([lo + (hi - lo) * (k + 0.5) / K for k in range(K)])
+    g = np.full(R * K, 1.0 / K)
+    graph = grid_csr(W, H)
+    for depth, e in enumerate(sorted(eps_list)):
+        g, th, rr = oracle.ts_solve(x, g, th, 1, R, K, 1, e, graph=graph,
+                                    ts=dict(skip_first_theta=1), maxit=100000, stop_difff=0,
+                                    stop_kkt=1, kkt_tol=1e-12, kkt_scale=1.0, nthreads=8,
+                                    gd_centered=1, gd_certified=1, dad_edges=1, g_recompute=1)
+        pre = os.path.join(res, f"usi_epssqr{e:g}")
+        gk = petsc_io.read_vec(pre + "_gamma.bin")
+        gd = petsc_io.permute_trk(gk, 1, R, K)  # file layout (k-major) -> node-major
+        rec = petsc_io.read_vec(pre + "_recovered.bin")
+        print(f"eps {e}: oracle it {rr.it} L {rr.L!r} theta {th} gamma err "
+              f"{np.abs(gd - g).max():.2e}", flush=True)
+        assert np.abs(gd - g).max() <= 1e-6
+        np.testing.assert_allclose(rec, gd.reshape(R, K) @ th, atol=1e-7)
+        assert os.path.exists(pre + "_original.bin") == (depth == 0)
+        if depth == 0:
+            np.testing.assert_array_equal(petsc_io.read_vec(pre + "_original.bin"), x)
+        thetas = open(short).read().strip().splitlines()[depth + 1].split(",")[5:5 + K]
+        # printed with the stream's default 6 significant digits
+        np.testing.assert_allclose([float(v) for v in thetas], th, rtol=1e-5)

@@ -457,3 +457,45 @@ def test_tiny_and_wide_problems(ctx, oracle, prec_name):
             assert r.status == 0 and r.kkt <= 1e-10, (T, K, r.kkt)
             assert G.min() >= 0.0 and np.max(np.abs(G.sum(axis=1) - 1.0)) <= 1e-12, (T, K)
             assert np.max(np.abs(g - ref)) <= 1e-6, (T, K, np.max(np.abs(g - ref)))
+
+
+@pytest.mark.parametrize("case", ["1d_fp64", "1d_fp32", "image", "fem"])
+def test_set_data_equals_a_fresh_problem(ctx, case):
+    """pasc_set_data replaces the data as TSData::cutdata / scaledata edit the datavector
+    (tsdata.cpp:384-478): theta steps and solves afterwards are bitwise those of a problem created
+    with the new data (1-D FP64 and FP32-bulk, an image with a held-out mask, a FEM-reduced
+    series)."""
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Problem, generate_markov, generate_mask, generate_rect_image
+    kw = {}
+    if case == "image":
+        w, h, K = 90, 40, 3
+        x1, col, _ = generate_rect_image(3, w, h, K, 6, 0.2)
+        x2, _, _ = generate_rect_image(4, w, h, K, 6, 0.2)
+        kw = dict(T=1, xdim=3, grid_width=w, heldout=generate_mask(5, w * h, 0.1),
+                  precision=capi.FP64)
+        theta = col
+    else:
+        K = 3
+        x1, _ = generate_markov(6, 20000, [0.0, 1.0, 3.0], 1e-3, 0.5)
+        x2 = np.clip(x1, 0.0, 2.5)  # cutdata(0, 2.5)
+        kw = dict(precision=capi.FP32_BULK if case == "1d_fp32" else capi.FP64)
+        if case == "fem":
+            kw.update(fem_reduce=0.5, fem_type=capi.FEM_HAT)
+        theta = np.array([0.2, 1.1, 2.6])
+    outs = []
+    for replace in (True, False):
+        P = Problem(ctx, x1 if replace else x2, K, epssqr=5.0, **kw)
+        if replace:
+            P.set_data(x2)
+        P.set_theta(theta)
+        P.set_gamma_uniform()
+        r = P.gamma_solve(maxit=300, stop_mask=0, polish_maxit=100)
+        P.theta_update()
+        r2 = P.gamma_solve(maxit=200, stop_mask=0, polish_maxit=100)
+        outs.append((P.get_gamma(), P.get_theta(), r.fx, r2.fx))
+        P.close()
+    print(f"{case}: f {outs[0][2]!r} {outs[1][2]!r}", flush=True)
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+    assert outs[0][2] == outs[1][2] and outs[0][3] == outs[1][3]

@@ -86,16 +86,17 @@ __device__ __forceinline__ void load_chunk(const S *__restrict__ row, int64_t t0
 }
 
 // Whole-wave lane shifts by one lane (DPP wave_shr:1 / wave_shl:1, VALU moves): lane i gets
-// lane i-1 (from_left) or lane i+1 (from_right); the lane without a source gets 0.
+// lane i-1 (from_left) or lane i+1 (from_right); the lane without a source gets 0 (the
+// bound-control bit: no register has to be set to 0 before each move).
 template <int CTRL>
 __device__ __forceinline__ float dpp_shift(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, true));
 }
 template <int CTRL>
 __device__ __forceinline__ double dpp_shift(double v) {
     const long long u = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(u & 0xffffffffll), CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, 0xf, 0xf, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(u & 0xffffffffll), CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), CTRL, 0xf, 0xf, true);
     return __longlong_as_double(((long long)(unsigned)hi << 32) | (long long)(unsigned)lo);
 }
 constexpr int DPP_WAVE_SHR1 = 0x138;  // lane i <- lane i-1

@@ -153,3 +153,52 @@ def test_cpp_image_driver_on_reference_image(oracle, tmp_path):
         thetas = open(short).read().strip().splitlines()[depth + 1].split(",")[5:5 + K]
         # printed with the stream's default 6 significant digits
         np.testing.assert_allclose([float(v) for v in thetas], th, rtol=1e-5)
+
+
+def test_cpp_image_driver_scaledata_cutgamma_given_theta(oracle, tmp_path):
+    """test_image with --test_scaledata (data and theta mapped from [0, 1] to [-1, 1] before each
+    solve and back before saving, tsdata.cpp:511-545), --test_cutgamma (gamma set to the
+    indicator of its largest component, tsdata.cpp:196-232) and a given theta (never updated).
+    As in test_image.cpp:183-236 the given theta is set after the data were scaled, so it is
+    taken in the scaled units and unscaled with the data after the solve.  The saved gamma is
+    the one-hot argmax of the oracle's gamma QP on the scaled data with that theta, and the
+    recovered image uses the unscaled theta."""
+    from pasc_inference_amd import petsc_io
+    W, H, K, eps = 250, 150, 2, 0.1
+    R = W * H
+    img = os.path.join(DATA, "usi_250_150_02.bin")
+    res = str(tmp_path / "results")
+    th = np.array([-0.2, 0.0])  # in the scaled units: 0.4 and 0.5 of the [0, 1] data
+    cmd = [IMAGE_DRIVER, f"--test_image_filename={img}", f"--test_width={W}",
+           f"--test_height={H}", f"--test_K={K}", f"--test_results_dir={res}",
+           "--test_image_out=usi", f"--test_shortinfo_filename={tmp_path / 'short.txt'}",
+           "--test_scaledata=1", "--test_cutgamma=1", "--test_Theta=-0.2", "--test_Theta=0",
+           f"--test_epssqr={eps}", "--spgqpsolver_stop_difff=0", "--spgqpsolver_stop_kkt=1",
+           "--spgqpsolver_kkt_tol=1e-11", "--spgqpsolver_maxit=100000"]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    x = np.clip(petsc_io.read_vec(img), 0.0, 1.0)  # cutdata(0, 1) is on by default
+    xs = 2.0 * x - 1.0                             # scaledata(-1, 1, 0, 1)
+    b = oracle.assemble_b(xs, th, 1, R, K)
+    g, _ = oracle.spg_solve(np.full(R * K, 1.0 / K), b, 1, R, K, eps, graph=grid_csr(W, H),
+                            maxit=200000, stop_difff=0, stop_kkt=1, kkt_tol=1e-12,
+                            kkt_scale=1.0, nthreads=8, gd_centered=1, gd_certified=1,
+                            dad_edges=1, g_recompute=1)
+    g = g.reshape(R, K)
+    # nodes whose two components are within the solves' difference of a tie are left out
+    clear = np.abs(g[:, 0] - g[:, 1]) > 1e-6
+    onehot = np.eye(K)[g.argmax(1)]
+    gd = petsc_io.permute_trk(petsc_io.read_vec(os.path.join(res, f"usi_epssqr{eps:g}_gamma.bin")),
+                              1, R, K).reshape(R, K)
+    print(f"one-hot nodes {np.mean(np.isin(gd, (0.0, 1.0)).all(1)):.4f}, "
+          f"argmax agreement {np.mean((gd == onehot).all(1)[clear]):.6f} on {clear.sum()} nodes",
+          flush=True)
+    assert np.isin(gd, (0.0, 1.0)).all() and np.all(gd.sum(1) == 1.0)
+    assert np.array_equal(gd[clear], onehot[clear])
+    rec = petsc_io.read_vec(os.path.join(res, f"usi_epssqr{eps:g}_recovered.bin"))
+    np.testing.assert_allclose(rec, gd @ ((th + 1.0) / 2.0), atol=1e-12)  # scaledata(0, 1, -1, 1)
+    # the data went to [-1, 1] and back: equal up to the rounding of the two maps
+    np.testing.assert_allclose(
+        petsc_io.read_vec(os.path.join(res, f"usi_epssqr{eps:g}_original.bin")), x, rtol=0,
+        atol=1e-15)

@@ -21,7 +21,7 @@ template <> class TSData<HipVector> {
            int precision = PASC_FP64, const std::vector<uint8_t> *heldout = nullptr,
            double fem_reduce = 1.0, int fem_type = PASC_FEM_HAT, int64_t R = 1,
            int32_t grid_width = 0)
-        : ctx_(ctx), x_(x), K_(K), xdim_(xdim), R_(R) {
+        : ctx_(ctx), x_(x), R_(R), K_(K), xdim_(xdim) {
         T_ = (int64_t)x.size() / ((int64_t)xdim * R);
         pasc_problem_desc d;
         std::memset(&d, 0, sizeof(d));

@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--no-graph-c5", action="store_true",
                     help="skip the image-graph solve reported beside the headline")
     ap.add_argument("--c5-eps", type=float, default=10.0)
+    ap.add_argument("--no-adjoint-c4", action="store_true",
+                    help="skip the differentiation workload reported beside the headline")
     ap.add_argument("--scan-series", type=int, default=256)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--plain-per-cell", type=int, default=2,
@@ -362,6 +364,76 @@ def run_graph_c5(args, ctx):
                      "of the FP64 sums), so pass_us is the figure to compare")
 
 
+def run_adjoint_c4(args, ctx):
+    """The differentiation workload: T = 1e6, K = 3, levels {0, 1.5, 3}, switch 1e-3, noise 0.8,
+    seed 3, 10% of the steps held out, epsilon 30, theta at the levels.  Timed: the FP64 gamma
+    solve to KKT 1e-12 from gamma0 = 1/K (graph-captured, the residual on every 4th iteration)
+    and the adjoint (projected CG on the active face to 1e-12; the second of two calls, the
+    first also allocates the work vectors), which gives the gradient of the
+    held-out squared reconstruction error with respect to log epsilon and theta.  Checked in the
+    same run against central differences (h = 1e-4) of that loss over the same solves.
+    Reported beside the headline, not as it."""
+    from pasc_inference_amd import capi
+    from pasc_inference_amd.solver import Problem, SIGNAL_C4, generate_markov, generate_mask
+    c = SIGNAL_C4
+    T = c["T"]
+    x, _ = generate_markov(c["seed"], T, c["levels"], c["p_switch"], c["sigma"])
+    mask = generate_mask(c["seed"], T, c["heldout"])
+    theta = np.array(c["levels"], dtype=np.float64)
+    K = theta.size
+    P = Problem(ctx, x, K, precision=capi.FP64, heldout=mask, epssqr=c["eps"])
+    params = dict(maxit=400000, stop_mask=capi.STOP_KKT, kkt_tol=1e-12, check_every=32,
+                  use_graph=1, kkt_every=4)
+
+    def loss(log_eps, th):
+        P.set_epssqr(float(np.exp(log_eps)))
+        P.set_theta(th)
+        P.set_gamma_uniform()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        r = P.gamma_solve(**params)
+        ctx.synchronize()
+        t = time.perf_counter() - t0
+        g = P.get_gamma().reshape(-1, K)
+        res = (x - g @ th)[mask == 1]
+        return float(np.sum(res ** 2)), r, t
+
+    le = float(np.log(c["eps"]))
+    loss(le, theta)  # warm-up
+    J, r, t_solve = loss(le, theta)
+    t_adj = []
+    for _ in range(2):  # the first call also allocates its work vectors (5 K T doubles)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        adj_loss, dle, dth, cg_it = P.adjoint(mask, cg_tol=1e-12, cg_maxit=200000)
+        ctx.synchronize()
+        t_adj.append(time.perf_counter() - t0)
+    log(args, f"adjoint c4: solve {t_solve:.4f}s it={r.it} adjoint {t_adj}s cg={cg_it}")
+    h = 1e-4
+    fd_le = (loss(le + h, theta)[0] - loss(le - h, theta)[0]) / (2 * h)
+    fd_th = []
+    for k in range(K):
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd_th.append((loss(le, tp)[0] - loss(le, tm)[0]) / (2 * h))
+    P.close()
+    return dict(workload=f"held-out loss gradient, T={T}, K={K}, levels {{0,1.5,3}}, switch 1e-3, "
+                         f"sigma 0.8, seed 3, 10% held out, epsilon {c['eps']:g}, FP64",
+                solve_s=t_solve, iterations=int(r.it), kkt=float(r.kkt), adjoint_s=t_adj[1],
+                adjoint_first_call_s=t_adj[0],
+                cg_iterations=int(cg_it), loss=J, adjoint_loss=adj_loss,
+                dloss_dlogeps=dle, fd_dloss_dlogeps=fd_le,
+                dlogeps_rel_diff=abs(dle - fd_le) / max(abs(fd_le), 1e-12),
+                dloss_dtheta=[float(v) for v in dth], fd_dloss_dtheta=fd_th,
+                dtheta_max_rel_diff=max(abs(a - b) / max(abs(b), 1e-12) for a, b in zip(dth, fd_th)),
+                max_scaled_diff=max(abs(a - b) / max(abs(b), 1e-3 * J)
+                                    for a, b in zip([dle] + list(dth), [fd_le] + fd_th)),
+                note="central differences with h = 1e-4 cross a few changes of the active face "
+                     "at this size; the GPU test bounds |FD - adjoint| by "
+                     "5e-4 max(|FD|, 1e-3 loss) (max_scaled_diff)")
+
+
 def run_scan(args, ctx):
     """The epsilon scan: every series x every epsilon, batched FP32-bulk (one engine, graph-
     captured, FP32 and FP64-polish iterations capped together at --scan-maxit) against the
@@ -545,6 +617,7 @@ def main():
             log(args, f"device copy kernel: {copy_gbps:.0f} GB/s")
             c2 = run_c2(args, ctx, tune_cache)
             graph = None if args.no_graph_c5 else run_graph_c5(args, ctx)
+            adjoint = None if args.no_adjoint_c4 else run_adjoint_c4(args, ctx)
             scan = None if args.no_scan else run_scan(args, ctx)
         traffic, traffic_err = (None, "skipped (--no-traffic)") if args.no_traffic else \
             measure_traffic(args, None if args.no_autotune else tune_cache)
@@ -608,6 +681,7 @@ def main():
         "fp32_polish_pass_avg_us": c2["pass32_polish"] * 1e6,
         "scan": scan,
         "graph_c5": graph,
+        "adjoint_c4": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -8,7 +8,9 @@
 // one CG solve of (P A P) lambda = P dJ/dgamma gives every parameter derivative:
 //   dJ/dlog(eps) = -lambda' A gamma                      (A = eps/T A0, so dA/dlog eps = A)
 //   dJ/dtheta_k  = dJ/dtheta_k|direct + sum_{t with data} lambda_kt * 2 (x_t - theta_k) / T
-// All vectors are FP64, k-major with row stride ld, one thread per time step.
+// All vectors are FP64, k-major with row stride ld, one thread per time step.  The CG scalars
+// (p'APp, r'r, the step and the update factor) are reduced and formed on the device, so an
+// iteration needs no host round trip; the host reads the stop flag every CG_POLL iterations.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -42,6 +44,16 @@ struct AdjArgs {
 };
 
 constexpr int NTH = PASC_MAX_K * 3;  // theta entries
+
+// Scalars of the CG iteration on the device.  adj_cg_step and adj_cg_rr form them from the
+// block partials of the kernel before; the vector kernels read step and b from here and do
+// nothing once done is set.
+struct CgState {
+    double rr, rr0, step, b, tol2;
+    int32_t it, maxit, done;
+};
+constexpr int CG_POLL = 16;  // CG iterations launched between host reads of the stop flag
+constexpr int RNT = 1024;    // threads of the one-workgroup partial reductions
 
 // (A0 v)_n of one row
 __device__ inline double adj_lap(const AdjArgs &a, const double *row, int64_t n) {
@@ -115,10 +127,51 @@ __global__ __launch_bounds__(ANT) void adj_init(AdjArgs a, double *r, double *pa
     block_sum<2 + NTH>(v, lds, part);
 }
 
+// Sum of part[0 .. blocks) in a fixed order (thread i adds blocks i, i + RNT, ..., then the
+// workgroup's tree); the result is valid in thread 0.
+__device__ inline double reduce_partials(const double *part, int blocks, double *lds) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < blocks; b += RNT) v += part[b];
+    v = wave_sum(v);
+    if ((threadIdx.x & (WAVE - 1)) == 0) lds[threadIdx.x / WAVE] = v;
+    __syncthreads();
+    double sum = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < RNT / WAVE; i++) sum += lds[i];
+    return sum;
+}
+
+// after adj_matvec: step = r'r / p'APp, or the stop when p'APp <= 0 (p in the null space of
+// P A P, where the system is consistent)
+__global__ __launch_bounds__(RNT) void adj_cg_step(const double *part, int blocks, CgState *cs) {
+    __shared__ double lds[RNT / WAVE];
+    if (cs->done) return;
+    const double pq = reduce_partials(part, blocks, lds);
+    if (threadIdx.x == 0) {
+        if (!(pq > 0.0)) cs->done = 1;
+        else cs->step = cs->rr / pq;
+    }
+}
+
+// after adj_axpy: b = r'r (new) / r'r (old), the iteration counted, and the stop once r'r is
+// below tol^2 r'r (initial) or the iterations reach maxit
+__global__ __launch_bounds__(RNT) void adj_cg_rr(const double *part, int blocks, CgState *cs) {
+    __shared__ double lds[RNT / WAVE];
+    if (cs->done) return;
+    const double rr_new = reduce_partials(part, blocks, lds);
+    if (threadIdx.x == 0) {
+        cs->b = rr_new / cs->rr;
+        cs->rr = rr_new;
+        cs->it += 1;
+        if (!(cs->it < cs->maxit && rr_new > cs->tol2 * cs->rr0 && rr_new > 0.0)) cs->done = 1;
+    }
+}
+
 // q = P A P p (p already in range(P)); partial p'q
 __global__ __launch_bounds__(ANT) void adj_matvec(AdjArgs a, const double *p, double *q,
-                                                  double *part) {
+                                                  double *part, const CgState *cs) {
     __shared__ double lds[ANT / WAVE];
+    if (cs->done) return;
     const int64_t t = (int64_t)blockIdx.x * ANT + threadIdx.x;
     double v[1] = {0.0};
     if (t < a.T) {
@@ -137,11 +190,13 @@ __global__ __launch_bounds__(ANT) void adj_matvec(AdjArgs a, const double *p, do
     block_sum<1>(v, lds, part);
 }
 
-// lam += s p; r -= s q; partial r'r
-__global__ __launch_bounds__(ANT) void adj_axpy(AdjArgs a, double s, const double *p,
+// lam += s p; r -= s q with s the step; partial r'r
+__global__ __launch_bounds__(ANT) void adj_axpy(AdjArgs a, const CgState *cs, const double *p,
                                                 const double *q, double *lam, double *r,
                                                 double *part) {
     __shared__ double lds[ANT / WAVE];
+    if (cs->done) return;
+    const double s = cs->step;
     const int64_t t = (int64_t)blockIdx.x * ANT + threadIdx.x;
     double v[1] = {0.0};
     if (t < a.T) {
@@ -155,8 +210,11 @@ __global__ __launch_bounds__(ANT) void adj_axpy(AdjArgs a, double s, const doubl
     block_sum<1>(v, lds, part);
 }
 
-__global__ __launch_bounds__(ANT) void adj_pupdate(AdjArgs a, double b, const double *r,
+// p = r + b p
+__global__ __launch_bounds__(ANT) void adj_pupdate(AdjArgs a, const CgState *cs, const double *r,
                                                    double *p) {
+    if (cs->done) return;
+    const double b = cs->b;
     const int64_t t = (int64_t)blockIdx.x * ANT + threadIdx.x;
     if (t >= a.T) return;
     for (int k = 0; k < a.K; k++) {
@@ -213,11 +271,13 @@ int engine_adjoint(Engine &e, int inst, const uint8_t *mask_host, double cg_tol,
     double *buf = nullptr;
     uint8_t *fr = nullptr, *mk = nullptr;
     double *part = nullptr;
+    CgState *d_cs = nullptr;
     // gamma (FP64 copy when the state is FP32), lambda, r, p, q
     HIP_TRY(hipMalloc(&buf, sizeof(double) * 5 * (size_t)n));
     HIP_TRY(hipMalloc(&fr, (size_t)n));
     HIP_TRY(hipMalloc(&mk, (size_t)T));
     HIP_TRY(hipMalloc(&part, sizeof(double) * (size_t)blocks * (2 + NTH)));
+    HIP_TRY(hipMalloc(&d_cs, sizeof(CgState)));
     double *gam = buf, *lam = buf + n, *r = buf + 2 * n, *p = buf + 3 * n, *q = buf + 4 * n;
     int rc = PASC_OK;
     auto cleanup = [&]() {
@@ -225,6 +285,7 @@ int engine_adjoint(Engine &e, int inst, const uint8_t *mask_host, double cg_tol,
         (void)hipFree(fr);
         (void)hipFree(mk);
         (void)hipFree(part);
+        (void)hipFree(d_cs);
     };
     const unsigned cb = (unsigned)((n + 255) / 256);
     if (e.cur64) {
@@ -259,25 +320,30 @@ int engine_adjoint(Engine &e, int inst, const uint8_t *mask_host, double cg_tol,
     HIP_TRY(hipGetLastError());
     if ((rc = sum_partials(s, part, blocks, 2 + NTH, v))) { cleanup(); return rc; }
     *loss = v[0];
-    double rr = v[1];
-    const double rr0 = rr;
     std::vector<double> direct(v.begin() + 2, v.begin() + 2 + D.K * D.xdim);
     HIP_TRY(hipMemcpyAsync(p, r, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
-    int it = 0;
-    // conjugate gradients on P A P (symmetric positive semi-definite on range(P))
-    while (it < cg_maxit && rr > cg_tol * cg_tol * rr0 && rr > 0.0) {
-        hipLaunchKernelGGL(adj_matvec, dim3(blocks), dim3(ANT), 0, s, a, p, q, part);
-        if ((rc = sum_partials(s, part, blocks, 1, v))) { cleanup(); return rc; }
-        const double pq = v[0];
-        if (!(pq > 0.0)) break;  // p in the null space of P A P: the system is consistent there
-        const double step = rr / pq;
-        hipLaunchKernelGGL(adj_axpy, dim3(blocks), dim3(ANT), 0, s, a, step, p, q, lam, r, part);
-        if ((rc = sum_partials(s, part, blocks, 1, v))) { cleanup(); return rc; }
-        const double rr_new = v[0];
-        hipLaunchKernelGGL(adj_pupdate, dim3(blocks), dim3(ANT), 0, s, a, rr_new / rr, r, p);
-        rr = rr_new;
-        it++;
+    // conjugate gradients on P A P (symmetric positive semi-definite on range(P)), the scalars
+    // on the device; the host reads the stop flag every CG_POLL iterations
+    CgState h0{};  // the initial state; stays alive until the copies below have completed
+    h0.rr = h0.rr0 = v[1];
+    h0.tol2 = cg_tol * cg_tol;
+    h0.maxit = cg_maxit;
+    h0.done = (cg_maxit > 0 && h0.rr > h0.tol2 * h0.rr0 && h0.rr > 0.0) ? 0 : 1;
+    HIP_TRY(hipMemcpyAsync(d_cs, &h0, sizeof(CgState), hipMemcpyHostToDevice, s));
+    CgState h = h0;
+    for (int launched = 0; !h.done && launched <= cg_maxit + CG_POLL; launched += CG_POLL) {
+        for (int i = 0; i < CG_POLL; i++) {
+            hipLaunchKernelGGL(adj_matvec, dim3(blocks), dim3(ANT), 0, s, a, p, q, part, d_cs);
+            hipLaunchKernelGGL(adj_cg_step, dim3(1), dim3(RNT), 0, s, part, blocks, d_cs);
+            hipLaunchKernelGGL(adj_axpy, dim3(blocks), dim3(ANT), 0, s, a, d_cs, p, q, lam, r, part);
+            hipLaunchKernelGGL(adj_cg_rr, dim3(1), dim3(RNT), 0, s, part, blocks, d_cs);
+            hipLaunchKernelGGL(adj_pupdate, dim3(blocks), dim3(ANT), 0, s, a, d_cs, r, p);
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(&h, d_cs, sizeof(CgState), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
     }
+    const int it = h.it;
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(adj_final, dim3(blocks), dim3(ANT), 0, s, a, lam, part);
     if ((rc = sum_partials(s, part, blocks, 1 + NTH, v))) { cleanup(); return rc; }

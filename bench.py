@@ -80,6 +80,8 @@ def parse():
                     help="run only the batched scan (for rocprofv3 traces of the batched engine)")
     ap.add_argument("--graph-profile", action="store_true",
                     help="run only the image-graph solve (for rocprofv3 traces of the graph kernels)")
+    ap.add_argument("--adjoint-profile", action="store_true",
+                    help="run only the differentiation workload (for rocprofv3 traces of the adjoint)")
     ap.add_argument("--tune-cache", default=None,
                     help="autotuner cache file: launch shapes recorded there are reused, new ones appended")
     ap.add_argument("--verbose", action="store_true")
@@ -322,7 +324,7 @@ def run_graph_c5(args, ctx):
     P = Problem(ctx, x, K, T=1, xdim=3, grid_width=w, precision=capi.FP64, epssqr=args.c5_eps)
     P.set_theta(col.ravel().copy())
     params = dict(maxit=200000, stop_mask=capi.STOP_KKT, kkt_tol=args.kkt, check_every=32,
-                  use_graph=1, kkt_every=4)
+                  use_graph=0 if args.no_graph else 1, kkt_every=4)
     times, r = _timed_solves(args, P, params, "graph (image) fp64 solve")
     g = P.get_gamma().reshape(-1, K)
     acc = float(np.mean(g.argmax(1) == lab))
@@ -383,7 +385,7 @@ def run_adjoint_c4(args, ctx):
     K = theta.size
     P = Problem(ctx, x, K, precision=capi.FP64, heldout=mask, epssqr=c["eps"])
     params = dict(maxit=400000, stop_mask=capi.STOP_KKT, kkt_tol=1e-12, check_every=32,
-                  use_graph=1, kkt_every=4)
+                  use_graph=0 if args.no_graph else 1, kkt_every=4)
 
     def loss(log_eps, th):
         P.set_epssqr(float(np.exp(log_eps)))
@@ -603,6 +605,11 @@ def main():
         from pasc_inference_amd.solver import Context
         with Context(0) as ctx:
             print(json.dumps(run_graph_c5(args, ctx)))
+        return
+    if args.adjoint_profile:
+        from pasc_inference_amd.solver import Context
+        with Context(0) as ctx:
+            print(json.dumps(run_adjoint_c4(args, ctx)))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path "

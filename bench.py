@@ -369,7 +369,8 @@ def run_graph_c5(args, ctx):
 def run_adjoint_c4(args, ctx):
     """The differentiation workload: T = 1e6, K = 3, levels {0, 1.5, 3}, switch 1e-3, noise 0.8,
     seed 3, 10% of the steps held out, epsilon 30, theta at the levels.  Timed: the FP64 gamma
-    solve to KKT 1e-12 from gamma0 = 1/K (graph-captured, the residual on every 4th iteration)
+    solve to KKT 1e-12 from gamma0 = 1/K (autotuned launch shape, graph-captured, the residual
+    on every 4th iteration)
     and the adjoint (projected CG on the active face to 1e-12; the second of two calls, the
     first also allocates the work vectors), which gives the gradient of the
     held-out squared reconstruction error with respect to log epsilon and theta.  Checked in the
@@ -384,6 +385,9 @@ def run_adjoint_c4(args, ctx):
     theta = np.array(c["levels"], dtype=np.float64)
     K = theta.size
     P = Problem(ctx, x, K, precision=capi.FP64, heldout=mask, epssqr=c["eps"])
+    P.set_theta(theta)
+    P.set_gamma_uniform()
+    shape = None if args.no_autotune else P.autotune(args.tune_cache)
     params = dict(maxit=400000, stop_mask=capi.STOP_KKT, kkt_tol=1e-12, check_every=32,
                   use_graph=0 if args.no_graph else 1, kkt_every=4)
 
@@ -422,7 +426,8 @@ def run_adjoint_c4(args, ctx):
     P.close()
     return dict(workload=f"held-out loss gradient, T={T}, K={K}, levels {{0,1.5,3}}, switch 1e-3, "
                          f"sigma 0.8, seed 3, 10% held out, epsilon {c['eps']:g}, FP64",
-                solve_s=t_solve, iterations=int(r.it), kkt=float(r.kkt), adjoint_s=t_adj[1],
+                launch_shape=shape, solve_s=t_solve, iterations=int(r.it), kkt=float(r.kkt),
+                adjoint_s=t_adj[1],
                 adjoint_first_call_s=t_adj[0],
                 cg_iterations=int(cg_it), loss=J, adjoint_loss=adj_loss,
                 dloss_dlogeps=dle, fd_dloss_dlogeps=fd_le,

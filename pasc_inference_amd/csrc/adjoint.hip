@@ -63,6 +63,15 @@ __device__ inline double adj_lap(const AdjArgs &a, const double *row, int64_t n)
     return l;
 }
 
+// (A0 v)_n of one row of v = rv + b pv, formed at n and at its neighbours
+__device__ inline double adj_lap_upd(const AdjArgs &a, const double *rv, const double *pv,
+                                     double b, int64_t n) {
+    const double c = rv[n] + b * pv[n];
+    double l = 0.0;
+    for_neighbours(a.G, n, [&](int64_t n2, double w) { l += w * (c - (rv[n2] + b * pv[n2])); });
+    return l;
+}
+
 template <int NV>
 __device__ inline void block_sum(double (&v)[NV], double *lds, double *out) {
     const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
@@ -167,24 +176,31 @@ __global__ __launch_bounds__(RNT) void adj_cg_rr(const double *part, int blocks,
     }
 }
 
-// q = P A P p (p already in range(P)); partial p'q
-__global__ __launch_bounds__(ANT) void adj_matvec(AdjArgs a, const double *p, double *q,
-                                                  double *part, const CgState *cs) {
+// The search direction p = r + b pold of this step (b = 0 on the first, where pold = r), written
+// to pnew, and q = P A P p (p in range(P)); partial p'q.  pnew and pold are two buffers, since
+// the neighbours' pold are read while other workgroups write.
+__global__ __launch_bounds__(ANT) void adj_matvec(AdjArgs a, const double *r, const double *pold,
+                                                  double *pnew, double *q, double *part,
+                                                  const CgState *cs) {
     __shared__ double lds[ANT / WAVE];
     if (cs->done) return;
+    const double b = cs->b;
     const int64_t t = (int64_t)blockIdx.x * ANT + threadIdx.x;
     double v[1] = {0.0};
     if (t < a.T) {
-        double w[PASC_MAX_K];
+        double w[PASC_MAX_K], pn[PASC_MAX_K];
         bool fr[PASC_MAX_K];
         for (int k = 0; k < a.K; k++) {
-            w[k] = a.alpha * adj_lap(a, p + k * a.ld, t);
+            const double *rv = r + k * a.ld, *pv = pold + k * a.ld;
+            pn[k] = rv[t] + b * pv[t];
+            w[k] = a.alpha * adj_lap_upd(a, rv, pv, b, t);
             fr[k] = a.freeset[k * a.ld + t] != 0;
         }
         project_face(w, fr, a.K);
         for (int k = 0; k < a.K; k++) {
+            pnew[k * a.ld + t] = pn[k];
             q[k * a.ld + t] = w[k];
-            v[0] += p[k * a.ld + t] * w[k];
+            v[0] += pn[k] * w[k];
         }
     }
     block_sum<1>(v, lds, part);
@@ -208,19 +224,6 @@ __global__ __launch_bounds__(ANT) void adj_axpy(AdjArgs a, const CgState *cs, co
         }
     }
     block_sum<1>(v, lds, part);
-}
-
-// p = r + b p
-__global__ __launch_bounds__(ANT) void adj_pupdate(AdjArgs a, const CgState *cs, const double *r,
-                                                   double *p) {
-    if (cs->done) return;
-    const double b = cs->b;
-    const int64_t t = (int64_t)blockIdx.x * ANT + threadIdx.x;
-    if (t >= a.T) return;
-    for (int k = 0; k < a.K; k++) {
-        const int64_t i = k * a.ld + t;
-        p[i] = r[i] + b * p[i];
-    }
 }
 
 // partials: [0] lambda' A gamma, [1 + k*xdim + c] sum_{n with data} lambda_kn 2 (x_nc - theta_kc) bscale
@@ -272,13 +275,14 @@ int engine_adjoint(Engine &e, int inst, const uint8_t *mask_host, double cg_tol,
     uint8_t *fr = nullptr, *mk = nullptr;
     double *part = nullptr;
     CgState *d_cs = nullptr;
-    // gamma (FP64 copy when the state is FP32), lambda, r, p, q
-    HIP_TRY(hipMalloc(&buf, sizeof(double) * 5 * (size_t)n));
+    // gamma (FP64 copy when the state is FP32), lambda, r, q and two buffers of p
+    HIP_TRY(hipMalloc(&buf, sizeof(double) * 6 * (size_t)n));
     HIP_TRY(hipMalloc(&fr, (size_t)n));
     HIP_TRY(hipMalloc(&mk, (size_t)T));
     HIP_TRY(hipMalloc(&part, sizeof(double) * (size_t)blocks * (2 + NTH)));
     HIP_TRY(hipMalloc(&d_cs, sizeof(CgState)));
-    double *gam = buf, *lam = buf + n, *r = buf + 2 * n, *p = buf + 3 * n, *q = buf + 4 * n;
+    double *gam = buf, *lam = buf + n, *r = buf + 2 * n, *q = buf + 3 * n;
+    double *pb[2] = {buf + 4 * n, buf + 5 * n};
     int rc = PASC_OK;
     auto cleanup = [&]() {
         (void)hipFree(buf);
@@ -299,7 +303,7 @@ int engine_adjoint(Engine &e, int inst, const uint8_t *mask_host, double cg_tol,
                            (const double *)e.d_gamma[par] + D.goff, gam, n);
     }
     HIP_TRY(hipMemcpyAsync(mk, mask_host, (size_t)T, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemsetAsync(buf + n, 0, sizeof(double) * 4 * (size_t)n, s));
+    HIP_TRY(hipMemsetAsync(buf + n, 0, sizeof(double) * 5 * (size_t)n, s));
     AdjArgs a;
     a.gam = gam;
     a.x = (const double *)e.d_x64 + D.xoff;
@@ -321,7 +325,7 @@ int engine_adjoint(Engine &e, int inst, const uint8_t *mask_host, double cg_tol,
     if ((rc = sum_partials(s, part, blocks, 2 + NTH, v))) { cleanup(); return rc; }
     *loss = v[0];
     std::vector<double> direct(v.begin() + 2, v.begin() + 2 + D.K * D.xdim);
-    HIP_TRY(hipMemcpyAsync(p, r, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(pb[0], r, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
     // conjugate gradients on P A P (symmetric positive semi-definite on range(P)), the scalars
     // on the device; the host reads the stop flag every CG_POLL iterations
     CgState h0{};  // the initial state; stays alive until the copies below have completed
@@ -333,11 +337,15 @@ int engine_adjoint(Engine &e, int inst, const uint8_t *mask_host, double cg_tol,
     CgState h = h0;
     for (int launched = 0; !h.done && launched <= cg_maxit + CG_POLL; launched += CG_POLL) {
         for (int i = 0; i < CG_POLL; i++) {
-            hipLaunchKernelGGL(adj_matvec, dim3(blocks), dim3(ANT), 0, s, a, p, q, part, d_cs);
+            // step launched + i reads p from pb[(launched + i) & 1] and writes the other
+            const double *pold = pb[i & 1];
+            double *pnew = pb[(i + 1) & 1];
+            hipLaunchKernelGGL(adj_matvec, dim3(blocks), dim3(ANT), 0, s, a, r, pold, pnew, q,
+                               part, d_cs);
             hipLaunchKernelGGL(adj_cg_step, dim3(1), dim3(RNT), 0, s, part, blocks, d_cs);
-            hipLaunchKernelGGL(adj_axpy, dim3(blocks), dim3(ANT), 0, s, a, d_cs, p, q, lam, r, part);
+            hipLaunchKernelGGL(adj_axpy, dim3(blocks), dim3(ANT), 0, s, a, d_cs, pnew, q, lam, r,
+                               part);
             hipLaunchKernelGGL(adj_cg_rr, dim3(1), dim3(RNT), 0, s, part, blocks, d_cs);
-            hipLaunchKernelGGL(adj_pupdate, dim3(blocks), dim3(ANT), 0, s, a, d_cs, r, p);
         }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(&h, d_cs, sizeof(CgState), hipMemcpyDeviceToHost, s));

@@ -114,6 +114,7 @@ def test_null_arguments_are_rejected():
     assert L.pasc_problem_create(None, None, None) == capi.PASC_ERR_ARG
     assert L.pasc_gamma_solve(None, None, None) != capi.PASC_OK
     assert L.pasc_project_simplex(None, None, 0, 3, 0, 0, 0) == capi.PASC_ERR_ARG
+    assert L.pasc_set_data(None, None) == capi.PASC_ERR_ARG
     p = capi.SpgParams()
     L.pasc_spg_default_params(C.byref(p))
     assert (p.maxit, p.eps, p.m, p.gamma, p.sigma1, p.sigma2, p.alphainit) == \

@@ -86,17 +86,7 @@ int main(int argc, char **argv) {
         std::cout << "- PROBLEM INFO ----------------------------" << std::endl;
         std::cout << " image = " << image_filename << ", " << width << " x " << height
                   << ", K = " << K << ", precision = " << precision << std::endl;
-        if (printstats) {
-            const auto &x = mydata.get_datavector();
-            double lo = x[0], hi = x[0], sum = 0.0;
-            for (double v : x) {
-                lo = std::min(lo, v);
-                hi = std::max(hi, v);
-                sum += v;
-            }
-            std::cout << " data: min = " << lo << ", max = " << hi << ", mean = " << sum / x.size()
-                      << std::endl;
-        }
+        if (printstats) mydata.printstats(std::cout);
         if (cutdata) mydata.cutdata(0, 1);
         if (scaledata) mydata.scaledata(-1, 1, 0, 1);
 

@@ -80,6 +80,27 @@ template <> class TSData<HipVector> {
         PASC_TRY(pasc_set_gamma(problem_, g.data(), PASC_LAYOUT_KMAJOR));
     }
 
+    // printstats (tsdata.cpp:310-383): length, blocks and the sum, max, min and mean of the data
+    void printstats(std::ostream &o) const {
+        double sum = 0.0, mx = x_.empty() ? 0.0 : x_[0], mn = mx;
+        for (double v : x_) {
+            sum += v;
+            mx = v > mx ? v : mx;
+            mn = v < mn ? v : mn;
+        }
+        const std::streamsize prec = o.precision();
+        o << std::setprecision(17) << "STATS: TSData" << std::endl;
+        o << " - total length:    " << std::setw(25) << x_.size() << std::endl;
+        o << " - nmb of blocks:   " << std::setw(25) << R_ * K_ << std::endl;
+        o << " - length of block: " << std::setw(25) << T_ << std::endl;
+        o << " - sum:             " << std::setw(25) << sum << std::endl;
+        o << " - max:             " << std::setw(25) << mx << std::endl;
+        o << " - min:             " << std::setw(25) << mn << std::endl;
+        o << " - avg:             " << std::setw(25) << (x_.empty() ? 0.0 : sum / x_.size())
+          << std::endl;
+        o << std::setprecision(prec);
+    }
+
     // cutdata (tsdata.cpp:461-478): clamp the data to [a, b]
     void cutdata(double a, double b) {
         for (double &v : x_) v = v > b ? b : (v < a ? a : v);

@@ -119,7 +119,7 @@ def test_cpp_image_driver_on_reference_image(oracle, tmp_path):
     short = str(tmp_path / "short" / "usi.txt")
     cmd = [IMAGE_DRIVER, f"--test_image_filename={img}", f"--test_width={W}",
            f"--test_height={H}", f"--test_K={K}", f"--test_results_dir={res}",
-           "--test_image_out=usi", f"--test_shortinfo_filename={short}",
+           "--test_image_out=usi", f"--test_shortinfo_filename={short}", "--test_printstats=1",
            "--spgqpsolver_stop_difff=0", "--spgqpsolver_stop_kkt=1",
            "--spgqpsolver_kkt_tol=1e-11", "--spgqpsolver_maxit=100000"]
     cmd += [f"--test_epssqr={e}" for e in eps_list]
@@ -127,6 +127,11 @@ def test_cpp_image_driver_on_reference_image(oracle, tmp_path):
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
     print(r.stdout[-3000:], r.stderr[-2000:])
     assert r.returncode == 0
+    # printstats of the data as loaded (before cutdata), tsdata.cpp:310-383
+    raw = petsc_io.read_vec(img)
+    m = re.search(r"- sum:\s+(\S+)\s+- max:\s+(\S+)\s+- min:\s+(\S+)", r.stdout)
+    assert m and np.isclose(float(m.group(1)), raw.sum(), rtol=1e-12)
+    assert (float(m.group(2)), float(m.group(3))) == (raw.max(), raw.min())
     # the oracle: cutdata(0, 1) (tsdata.cpp:461-478), theta0 spread over the data range, a
     # uniform gamma0 and a gamma step first, each epsilon from the previous gamma and theta
     x = np.clip(petsc_io.read_vec(img), 0.0, 1.0)
